@@ -1,0 +1,144 @@
+/*
+ * kronmult_amd.h -- C ABI of the MI355X (gfx950) batched Kronecker
+ * matrix-vector library.
+ *
+ *   Y(:,k) += kron(A1(k), ..., Ad(k)) * X(:,k),   k = 1..batchCount, d <= 6
+ *
+ * Conventions shared by every entry point (they are the reference's):
+ *   - column-major storage; in kron(A1,...,Ad) A1 acts on the slowest index
+ *     of X and Ad on the fastest (kroncommon.hpp:97-199);
+ *   - Y is accumulated (+=).  Wherever two batch entries may name the same Y
+ *     vector (xbatched, vbatched, kgemm with beta == 1) the update is an
+ *     element-wise device atomic add, as in the reference
+ *     (kgemm_nn.hpp:392-394);
+ *   - every data pointer and every pointer/leading-dimension array is device
+ *     memory owned by the caller; the library allocates nothing on these
+ *     paths and never synchronises the stream;
+ *   - X is never written and W is used only as scratch when a shape does not
+ *     fit on chip (the reference clobbers X in its batched layouts; callers
+ *     must still treat X/W contents as unspecified after a call there).
+ *   - stream is a hipStream_t (NULL = the null stream).
+ *
+ * All functions return a kron_status.  Argument errors are detected on the
+ * host before anything is launched; KRON_EHIP reports a failed launch.
+ */
+#ifndef KRONMULT_AMD_H
+#define KRONMULT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kron_status {
+    KRON_OK = 0,
+    KRON_EINVAL = 1,      /* invalid argument (d, n, sizes, null pointers) */
+    KRON_EWORKSPACE = 2,  /* workspace (W / pW / Wcapacity) too small */
+    KRON_EHIP = 3,        /* HIP runtime reported an error */
+    KRON_ENOTSUP = 4      /* shape outside every kernel in this build */
+} kron_status;
+
+typedef void *kron_stream_t; /* hipStream_t */
+
+const char *kron_status_string(kron_status s);
+/* Last HIP error code seen by the library on this thread (0 if none). */
+int kron_last_hip_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int kron_version(void);
+
+/*
+ * Contiguous batched layout.  Replaces kronmult{1..6}_batched
+ * (kronmult6_batched.cpp:5-25, kernel kronmult6_batched.hpp:14-94).
+ *   A: (n, n, d, batchCount), factor k of entry b at ((b*d)+k)*n*n
+ *   X, Y, W: (n^d, batchCount)
+ * W is scratch (needed only when n^d does not fit on chip).
+ */
+kron_status kron_batched_f64(int d, int n, const double *A, const double *X,
+                             double *Y, double *W, int batchCount,
+                             kron_stream_t stream);
+kron_status kron_batched_f32(int d, int n, const float *A, const float *X,
+                             float *Y, float *W, int batchCount,
+                             kron_stream_t stream);
+
+/*
+ * Pointer-array layout with shared Y.  Replaces kronmult{1..6}_xbatched
+ * (kronmult6_xbatched.cpp:5-39, body kronmult_xbatched.hpp:19-157).
+ *   Aarray[k + b*d] -> n x n factor k of entry b, leading dimension lda >= n
+ *   pX[b], pY[b]     -> vectors of n^d; pY entries may coincide (atomic add)
+ *   pW               -> scratch vectors of n^d: 2*subbatchCount of them when
+ *                       batchCount > 1, else 1 (kronmult_xbatched.hpp:125-126);
+ *                       used only when n^d does not fit on chip.
+ *   subbatchCount == 0 means max(1, batchCount/2) (kronmult_xbatched.hpp:73-75).
+ */
+kron_status kron_xbatched_f64(int d, int n, const double *const *Aarray, int lda,
+                              const double *const *pX, double *const *pY,
+                              double *const *pW, int batchCount,
+                              int subbatchCount, kron_stream_t stream);
+kron_status kron_xbatched_f32(int d, int n, const float *const *Aarray, int lda,
+                              const float *const *pX, float *const *pY,
+                              float *const *pW, int batchCount,
+                              int subbatchCount, kron_stream_t stream);
+
+/*
+ * Variable-shape layout: factor k is m[k] x n[k] with leading dimension m[k]
+ * (one shape set per call).  Replaces the kernel templates
+ * kronmult{1..6}_vbatched (kronmult6_vbatched.hpp:14-58, body
+ * kronmult_vbatched.hpp:19-248), which have no host wrapper.
+ *   Aarray[k + b*d], pX[b] (prod n), pY[b] (prod m, may coincide)
+ *   W: scratch of Wcapacity_bytes.  As in the reference
+ *   (kronmult_vbatched.hpp:136-157) the call needs room for one slot of
+ *   2*sizeXW elements; a smaller Wcapacity returns KRON_EWORKSPACE where the
+ *   reference would overrun.
+ */
+kron_status kron_vbatched_f64(int d, const int *m, const int *n,
+                              const double *const *Aarray,
+                              const double *const *pX, double *const *pY,
+                              double *W, size_t Wcapacity_bytes, int batchCount,
+                              kron_stream_t stream);
+kron_status kron_vbatched_f32(int d, const int *m, const int *n,
+                              const float *const *Aarray,
+                              const float *const *pX, float *const *pY,
+                              float *W, size_t Wcapacity_bytes, int batchCount,
+                              kron_stream_t stream);
+
+/*
+ * Batched small GEMM with per-entry pointers and leading dimensions (device
+ * arrays), uniform mm, nn, kk.  Replaces kgemm_nn_batched
+ * (kgemm_nn_batched.cpp:9-42) and kgemm_nt_batched (kgemm_nt_batched.cpp:8-41).
+ *   nn: C_b = alpha*A_b*B_b   + beta*C_b,  A mm x kk, B kk x nn, C mm x nn
+ *   nt: C_b = alpha*A_b*B_b^T + beta*C_b,  A mm x kk, B nn x kk, C mm x nn
+ * beta == 1: element-wise atomic add into C (C may be shared between entries);
+ * beta == 0: C is overwritten and never read;
+ * otherwise: C = beta*C + alpha*AB (kgemm_nn.hpp:391-400).
+ */
+kron_status kron_kgemm_nn_batched_f64(int mm, int nn, int kk, double alpha,
+                                      const double *const *Aarray, const int *ldAarray,
+                                      const double *const *Barray, const int *ldBarray,
+                                      double beta, double *const *Carray,
+                                      const int *ldCarray, int batchCount,
+                                      kron_stream_t stream);
+kron_status kron_kgemm_nt_batched_f64(int mm, int nn, int kk, double alpha,
+                                      const double *const *Aarray, const int *ldAarray,
+                                      const double *const *Barray, const int *ldBarray,
+                                      double beta, double *const *Carray,
+                                      const int *ldCarray, int batchCount,
+                                      kron_stream_t stream);
+kron_status kron_kgemm_nn_batched_f32(int mm, int nn, int kk, float alpha,
+                                      const float *const *Aarray, const int *ldAarray,
+                                      const float *const *Barray, const int *ldBarray,
+                                      float beta, float *const *Carray,
+                                      const int *ldCarray, int batchCount,
+                                      kron_stream_t stream);
+kron_status kron_kgemm_nt_batched_f32(int mm, int nn, int kk, float alpha,
+                                      const float *const *Aarray, const int *ldAarray,
+                                      const float *const *Barray, const int *ldBarray,
+                                      float beta, float *const *Carray,
+                                      const int *ldCarray, int batchCount,
+                                      kron_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KRONMULT_AMD_H */

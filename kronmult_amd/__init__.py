@@ -1,0 +1,214 @@
+"""kronmult_amd -- batched Kronecker-product matrix-vector multiply on MI355X.
+
+    Y(:,k) += kron(A1(k), ..., Ad(k)) * X(:,k),   d <= 6
+
+Python host mirror of the reference's entry points (hipdac-lab/kronmult:
+kronmult{1..6}_batched.cpp, kronmult{1..6}_xbatched.cpp, the
+kronmult{1..6}_vbatched kernels, kgemm_n{n,t}_batched.cpp) over the C ABI
+of libkronmult_amd.so (include/kronmult_amd.h).  Argument meaning and order
+follow the reference; every data argument is device memory, given as a torch
+tensor or as an integer device address.  Pointer arrays (Aarray, Xarray, ...
+of the pointer layouts) are device tensors of int64 addresses, as in the
+reference where the kernels dereference them on the device.
+
+Errors: the reference does not check arguments; here a non-OK status raises
+KronError before anything is launched.  There is no CPU fallback: when the
+HIP library is missing every call raises.
+"""
+import ctypes
+import functools
+import os
+
+__all__ = [
+    "KronError", "lib", "ptr_array",
+    "kronmult_batched", "kronmult_xbatched", "kronmult_vbatched",
+    "kgemm_nn_batched", "kgemm_nt_batched",
+] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkronmult_amd.so")
+
+KRON_OK, KRON_EINVAL, KRON_EWORKSPACE, KRON_EHIP, KRON_ENOTSUP = range(5)
+
+
+class KronError(RuntimeError):
+    """A kron_status other than KRON_OK."""
+
+    def __init__(self, fn, status, message):
+        super().__init__("%s: %s" % (fn, message))
+        self.status = status
+
+
+_c_int, _c_vp, _c_sz = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
+_c_ip = ctypes.POINTER(ctypes.c_int)
+
+_SIGS = {
+    "kron_batched_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp],
+    "kron_xbatched_%s": [_c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_vp],
+    "kron_vbatched_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int, _c_vp],
+}
+
+
+@functools.lru_cache(maxsize=1)
+def lib():
+    """Load libkronmult_amd.so (built in-tree by kronmult_amd.build)."""
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "kronmult_amd: %s is not built (run `python -m kronmult_amd.build`); "
+            "there is no fallback path" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    for sfx, real in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+        for name, args in _SIGS.items():
+            f = getattr(L, name % sfx)
+            f.argtypes = args
+            f.restype = _c_int
+        for tr in ("nn", "nt"):
+            f = getattr(L, "kron_kgemm_%s_batched_%s" % (tr, sfx))
+            f.argtypes = [_c_int, _c_int, _c_int, real, _c_vp, _c_vp, _c_vp, _c_vp, real,
+                          _c_vp, _c_vp, _c_int, _c_vp]
+            f.restype = _c_int
+    L.kron_status_string.argtypes = [_c_int]
+    L.kron_status_string.restype = ctypes.c_char_p
+    L.kron_last_hip_error.restype = _c_int
+    L.kron_version.restype = _c_int
+    return L
+
+
+def _addr(x):
+    """Device address of a tensor / int / None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _sfx(dtype):
+    import torch
+    if dtype in (torch.float64, "float64", "f64", "double"):
+        return "f64"
+    if dtype in (torch.float32, "float32", "f32", "float"):
+        return "f32"
+    raise TypeError("kronmult_amd supports float64 and float32, got %r" % (dtype,))
+
+
+def _stream(stream):
+    if stream is not None:
+        return stream if isinstance(stream, int) else stream.cuda_stream
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check(fn, st):
+    if st != KRON_OK:
+        L = lib()
+        msg = L.kron_status_string(st).decode()
+        if st == KRON_EHIP:
+            msg += " (hipError %d)" % L.kron_last_hip_error()
+        raise KronError(fn, st, msg)
+
+
+def ptr_array(items, device="cuda"):
+    """Device int64 tensor of addresses (tensors or ints) -- a pointer array."""
+    import torch
+    return torch.tensor([_addr(t) for t in items], dtype=torch.int64, device=device)
+
+
+def kronmult_batched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, stream=None):
+    """kronmult{d}_batched (kronmult6_batched.cpp:5-25).
+
+    Aarray (n,n,d,batch), Xarray/Yarray/Warray (n^d, batch), column-major.
+    Y += kron(A1..Ad) X per entry; X is read-only, W scratch."""
+    sfx = _sfx(Aarray.dtype)
+    f = getattr(lib(), "kron_batched_" + sfx)
+    _check("kronmult%d_batched" % d,
+           f(d, n, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(Warray), batchCount,
+             _stream(stream)))
+
+
+def kronmult_xbatched(d, n, Aarray, lda, Xarray, Yarray, Warray, batchCount,
+                      subbatchCount=0, dtype=None, stream=None):
+    """kronmult{d}_xbatched (kronmult6_xbatched.cpp:5-39).
+
+    Aarray: int64 device pointers, Aarray[k + b*d] -> n x n factor (ld lda);
+    Xarray/Yarray/Warray: int64 device pointers (Y entries may coincide)."""
+    import torch
+    sfx = _sfx(dtype if dtype is not None else torch.float64)
+    f = getattr(lib(), "kron_xbatched_" + sfx)
+    _check("kronmult%d_xbatched" % d,
+           f(d, n, _addr(Aarray), lda, _addr(Xarray), _addr(Yarray), _addr(Warray),
+             batchCount, subbatchCount, _stream(stream)))
+
+
+def kronmult_vbatched(m, n, Aarray, Xarray, Yarray, W, Wcapacity, batchCount,
+                      dtype=None, stream=None):
+    """kronmult{d}_vbatched (kronmult6_vbatched.hpp:14-58), d = len(m).
+
+    Factor k is m[k] x n[k] with leading dimension m[k]; W is a scratch buffer
+    of Wcapacity bytes."""
+    import torch
+    d = len(m)
+    if len(n) != d:
+        raise KronError("kronmult_vbatched", KRON_EINVAL, "len(m) != len(n)")
+    sfx = _sfx(dtype if dtype is not None else torch.float64)
+    f = getattr(lib(), "kron_vbatched_" + sfx)
+    mm = (ctypes.c_int * d)(*m)
+    nn = (ctypes.c_int * d)(*n)
+    _check("kronmult%d_vbatched" % d,
+           f(d, mm, nn, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(W), Wcapacity,
+             batchCount, _stream(stream)))
+
+
+def _kgemm(tr, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,
+           ldCarray, batchCount, dtype, stream):
+    import torch
+    sfx = _sfx(dtype if dtype is not None else torch.float64)
+    f = getattr(lib(), "kron_kgemm_%s_batched_%s" % (tr, sfx))
+    _check("kgemm_%s_batched" % tr,
+           f(mm, nn, kk, alpha, _addr(Aarray), _addr(ldAarray), _addr(Barray),
+             _addr(ldBarray), beta, _addr(Carray), _addr(ldCarray), batchCount,
+             _stream(stream)))
+
+
+def kgemm_nn_batched(mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,
+                     ldCarray, batchCount, dtype=None, stream=None):
+    """kgemm_nn_batched (kgemm_nn_batched.cpp:9-42): C_b = alpha A_b B_b + beta C_b."""
+    _kgemm("nn", mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,
+           ldCarray, batchCount, dtype, stream)
+
+
+def kgemm_nt_batched(mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,
+                     ldCarray, batchCount, dtype=None, stream=None):
+    """kgemm_nt_batched (kgemm_nt_batched.cpp:8-41): C_b = alpha A_b B_b^T + beta C_b."""
+    _kgemm("nt", mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,
+           ldCarray, batchCount, dtype, stream)
+
+
+def _bind(d):
+    def batched(n, Aarray, Xarray, Yarray, Warray, batchCount, stream=None):
+        return kronmult_batched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, stream)
+
+    def xbatched(n, Aarray, lda, Xarray, Yarray, Warray, batchCount, subbatchCount=0,
+                 dtype=None, stream=None):
+        return kronmult_xbatched(d, n, Aarray, lda, Xarray, Yarray, Warray, batchCount,
+                                 subbatchCount, dtype, stream)
+
+    def vbatched(*args, dtype=None, stream=None):
+        # (m1, n1, ..., md, nd, Aarray, pX, pY, W, Wcapacity, batchCount)
+        if len(args) != 2 * d + 6:
+            raise TypeError("kronmult%d_vbatched takes %d positional arguments" % (d, 2 * d + 6))
+        mn = args[:2 * d]
+        rest = args[2 * d:]
+        return kronmult_vbatched(list(mn[0::2]), list(mn[1::2]), *rest, dtype=dtype,
+                                 stream=stream)
+
+    for f, kind in ((batched, "batched"), (xbatched, "xbatched"), (vbatched, "vbatched")):
+        f.__name__ = "kronmult%d_%s" % (d, kind)
+        f.__doc__ = "kronmult%d_%s -- see kronmult_%s." % (d, kind, kind)
+    return batched, xbatched, vbatched
+
+
+for _d in range(1, 7):
+    (globals()["kronmult%d_batched" % _d], globals()["kronmult%d_xbatched" % _d],
+     globals()["kronmult%d_vbatched" % _d]) = _bind(_d)
+del _d

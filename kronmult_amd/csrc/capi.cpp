@@ -1,0 +1,341 @@
+// capi.cpp -- C ABI (include/kronmult_amd.h) and the drop-in C++ overloads
+// (include/kronmult.hpp).  Host code only: validates arguments, applies the
+// reference's workspace rules, and hands a KronArgs to the kernel launchers.
+#include <cstdio>
+#include <mutex>
+
+#include "kron_internal.hpp"
+#include "../../include/kronmult.hpp"
+
+namespace kron {
+
+static thread_local int g_last_hip_error = 0;
+void set_last_hip_error(int code) { g_last_hip_error = code; }
+
+int max_lds_bytes()
+{
+    static std::mutex mu;
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cached[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess || v <= 0)
+            v = 65536;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
+// n^d with overflow check against the reference's int sizes.
+static bool ipow_ok(int n, int d, long long &out)
+{
+    long long r = 1;
+    for (int k = 0; k < d; ++k) {
+        r *= n;
+        if (r > 0x7fffffffLL) return false;
+    }
+    out = r;
+    return true;
+}
+
+template <typename T>
+static kron_status batched_impl(int d, int n, const T *A, const T *X, T *Y, T *W, int batch,
+                                kron_stream_t stream)
+{
+    long long E = 0;
+    if (d < 1 || d > kMaxDim || n < 1 || batch < 0 || !ipow_ok(n, d, E)) return KRON_EINVAL;
+    if (batch == 0) return KRON_OK;
+    if (!A || !X || !Y) return KRON_EINVAL;
+    KronArgs<T> a;
+    a.d = d;
+    for (int k = 0; k < d; ++k) a.m[k] = a.n[k] = n;
+    a.lda = n;
+    a.layout = Layout::Batched;
+    a.A = A;
+    a.X = X;
+    a.Y = Y;
+    a.W = W;
+    a.scratch = Scratch::BatchedW;
+    a.y_shared = false;
+    a.batch = batch;
+    a.stream = static_cast<hipStream_t>(stream);
+    if (kron_scratch_elems(a) > 0 && !W) return KRON_EWORKSPACE;
+    return launch_kron(a);
+}
+
+template <typename T>
+static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
+                                 const T *const *pX, T *const *pY, T *const *pW, int batch,
+                                 int subbatch, kron_stream_t stream)
+{
+    long long E = 0;
+    if (d < 1 || d > kMaxDim || n < 1 || lda < n || batch < 0 || subbatch < 0 ||
+        !ipow_ok(n, d, E))
+        return KRON_EINVAL;
+    if (batch == 0) return KRON_OK;
+    if (!Aarray || !pX || !pY) return KRON_EINVAL;
+    // kronmult_xbatched.hpp:73-75
+    if (subbatch == 0) subbatch = (batch / 2 > 1) ? batch / 2 : 1;
+    KronArgs<T> a;
+    a.d = d;
+    for (int k = 0; k < d; ++k) a.m[k] = a.n[k] = n;
+    a.lda = lda;
+    a.layout = Layout::Ptr;
+    a.pA = Aarray;
+    a.pX = pX;
+    a.pY = pY;
+    a.pW = pW;
+    // kronmult_xbatched.hpp:125-126: slots pW[2*(iz-1)], pW[2*(iz-1)+1] for a
+    // sub-batch of subbatch entries; a single entry uses pW[0].
+    a.scratch = (batch > 1) ? Scratch::PtrSlots : Scratch::PtrSlot0;
+    a.nslots = (batch > 1) ? (subbatch < batch ? subbatch : batch) : 1;
+    a.y_shared = true;
+    a.batch = batch;
+    a.stream = static_cast<hipStream_t>(stream);
+    if (kron_scratch_elems(a) > 0 && !pW) return KRON_EWORKSPACE;
+    return launch_kron(a);
+}
+
+template <typename T>
+static kron_status vbatched_impl(int d, const int *m, const int *n, const T *const *Aarray,
+                                 const T *const *pX, T *const *pY, T *W, size_t Wcap,
+                                 int batch, kron_stream_t stream)
+{
+    if (d < 1 || d > kMaxDim || batch < 0 || !m || !n) return KRON_EINVAL;
+    int mm[kMaxDim], nn[kMaxDim];
+    for (int k = 0; k < kMaxDim; ++k) {
+        mm[k] = (k < d) ? m[k] : 1;
+        nn[k] = (k < d) ? n[k] : 1;
+        if (mm[k] < 1 || nn[k] < 1) return KRON_EINVAL;
+    }
+    // kronmult_vbatched.hpp:136-157
+    long long sizeX = 1;
+    for (int k = 0; k < kMaxDim; ++k) sizeX *= nn[k];
+    long long sizeW = 0;
+    for (int k = 0; k < kMaxDim - 1; ++k) {
+        long long s = 1;
+        for (int l = 0; l < kMaxDim; ++l) s *= (l <= k) ? mm[l] : nn[l];
+        if (s > sizeW) sizeW = s;
+    }
+    long long sizeY = 1;
+    for (int k = 0; k < kMaxDim; ++k) sizeY *= mm[k];
+    if (sizeX > 0x7fffffffLL || sizeW > 0x7fffffffLL || sizeY > 0x7fffffffLL)
+        return KRON_EINVAL;
+    const long long sizeXW = sizeX > sizeW ? sizeX : sizeW;
+    const size_t slot_bytes = 2 * (size_t)sizeXW * sizeof(T);
+    if (batch == 0) return KRON_OK;
+    if (!Aarray || !pX || !pY) return KRON_EINVAL;
+    // The reference clamps subbatch to 1 and overruns W when Wcap is too small.
+    if (Wcap < slot_bytes) return KRON_EWORKSPACE;
+    if (!W) return KRON_EINVAL;
+    long long subbatch = (long long)(Wcap / slot_bytes);
+    if (subbatch > batch) subbatch = batch;
+    if (subbatch < 1) subbatch = 1;
+    KronArgs<T> a;
+    a.d = d;
+    for (int k = 0; k < kMaxDim; ++k) {
+        a.m[k] = mm[k];
+        a.n[k] = nn[k];
+    }
+    a.lda = 0;  // ld_k = m_k (kronmult_vbatched.hpp:206-211)
+    a.layout = Layout::Ptr;
+    a.pA = Aarray;
+    a.pX = pX;
+    a.pY = pY;
+    a.Wbase = W;
+    a.slot_stride = 2 * (size_t)sizeXW;
+    a.nslots = (int)subbatch;
+    a.scratch = Scratch::Linear;
+    a.y_shared = true;
+    a.batch = batch;
+    a.stream = static_cast<hipStream_t>(stream);
+    return launch_kron(a);
+}
+
+template <typename T>
+static kron_status kgemm_impl(bool transB, int mm, int nn, int kk, T alpha,
+                              const T *const *A, const int *ldA, const T *const *B,
+                              const int *ldB, T beta, T *const *C, const int *ldC, int batch,
+                              kron_stream_t stream)
+{
+    if (mm < 0 || nn < 0 || kk < 0 || batch < 0) return KRON_EINVAL;
+    if (batch == 0 || mm == 0 || nn == 0) return KRON_OK;
+    if (!A || !ldA || !B || !ldB || !C || !ldC) return KRON_EINVAL;
+    return launch_kgemm<T>(transB, mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC, batch,
+                           static_cast<hipStream_t>(stream));
+}
+
+}  // namespace kron
+
+using namespace kron;
+
+extern "C" {
+
+const char *kron_status_string(kron_status s)
+{
+    switch (s) {
+    case KRON_OK: return "KRON_OK";
+    case KRON_EINVAL: return "KRON_EINVAL: invalid argument";
+    case KRON_EWORKSPACE: return "KRON_EWORKSPACE: workspace too small or missing";
+    case KRON_EHIP: return "KRON_EHIP: HIP runtime error";
+    case KRON_ENOTSUP: return "KRON_ENOTSUP: shape not supported";
+    }
+    return "unknown kron_status";
+}
+
+int kron_last_hip_error(void) { return g_last_hip_error; }
+int kron_version(void) { return 100; }
+
+kron_status kron_batched_f64(int d, int n, const double *A, const double *X, double *Y,
+                             double *W, int batch, kron_stream_t s)
+{
+    return batched_impl<double>(d, n, A, X, Y, W, batch, s);
+}
+kron_status kron_batched_f32(int d, int n, const float *A, const float *X, float *Y, float *W,
+                             int batch, kron_stream_t s)
+{
+    return batched_impl<float>(d, n, A, X, Y, W, batch, s);
+}
+kron_status kron_xbatched_f64(int d, int n, const double *const *A, int lda,
+                              const double *const *pX, double *const *pY, double *const *pW,
+                              int batch, int subbatch, kron_stream_t s)
+{
+    return xbatched_impl<double>(d, n, A, lda, pX, pY, pW, batch, subbatch, s);
+}
+kron_status kron_xbatched_f32(int d, int n, const float *const *A, int lda,
+                              const float *const *pX, float *const *pY, float *const *pW,
+                              int batch, int subbatch, kron_stream_t s)
+{
+    return xbatched_impl<float>(d, n, A, lda, pX, pY, pW, batch, subbatch, s);
+}
+kron_status kron_vbatched_f64(int d, const int *m, const int *n, const double *const *A,
+                              const double *const *pX, double *const *pY, double *W,
+                              size_t Wcap, int batch, kron_stream_t s)
+{
+    return vbatched_impl<double>(d, m, n, A, pX, pY, W, Wcap, batch, s);
+}
+kron_status kron_vbatched_f32(int d, const int *m, const int *n, const float *const *A,
+                              const float *const *pX, float *const *pY, float *W,
+                              size_t Wcap, int batch, kron_stream_t s)
+{
+    return vbatched_impl<float>(d, m, n, A, pX, pY, W, Wcap, batch, s);
+}
+
+#define KGEMM_C(NAME, T, TRANS)                                                            \
+    kron_status NAME(int mm, int nn, int kk, T alpha, const T *const *A, const int *ldA,   \
+                     const T *const *B, const int *ldB, T beta, T *const *C,               \
+                     const int *ldC, int batch, kron_stream_t s)                           \
+    {                                                                                      \
+        return kgemm_impl<T>(TRANS, mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC,       \
+                             batch, s);                                                    \
+    }
+KGEMM_C(kron_kgemm_nn_batched_f64, double, false)
+KGEMM_C(kron_kgemm_nt_batched_f64, double, true)
+KGEMM_C(kron_kgemm_nn_batched_f32, float, false)
+KGEMM_C(kron_kgemm_nt_batched_f32, float, true)
+#undef KGEMM_C
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Drop-in C++ overloads (null stream), reference signatures.
+// ---------------------------------------------------------------------------
+static thread_local kron_status g_last_status = KRON_OK;
+kron_status kronmult_last_status() { return g_last_status; }
+
+static void report(const char *fn, kron_status st)
+{
+    g_last_status = st;
+    if (st != KRON_OK) std::fprintf(stderr, "kronmult_amd: %s: %s\n", fn, kron_status_string(st));
+}
+
+#define DEF_BATCHED(D, T, SFX)                                                             \
+    void kronmult##D##_batched(int const n, T const Aarray_[], T Xarray_[], T Yarray_[],  \
+                               T Warray_[], int const batchCount)                          \
+    {                                                                                      \
+        report("kronmult" #D "_batched",                                                   \
+               kron_batched_##SFX(D, n, Aarray_, Xarray_, Yarray_, Warray_, batchCount,    \
+                                  nullptr));                                               \
+    }
+#define DEF_XBATCHED(D, T, SFX)                                                            \
+    void kronmult##D##_xbatched(int const n, T const *const Aarray_[], int const lda,     \
+                                T *Xarray_[], T *Yarray_[], T *Warray_[],                  \
+                                int const batchCount, int const subbatchCount)             \
+    {                                                                                      \
+        report("kronmult" #D "_xbatched",                                                  \
+               kron_xbatched_##SFX(D, n, Aarray_, lda, Xarray_, Yarray_, Warray_,          \
+                                   batchCount, subbatchCount, nullptr));                   \
+    }
+#define DEF_ALL(T, SFX)                                                                    \
+    DEF_BATCHED(1, T, SFX) DEF_BATCHED(2, T, SFX) DEF_BATCHED(3, T, SFX)                   \
+    DEF_BATCHED(4, T, SFX) DEF_BATCHED(5, T, SFX) DEF_BATCHED(6, T, SFX)                   \
+    DEF_XBATCHED(1, T, SFX) DEF_XBATCHED(2, T, SFX) DEF_XBATCHED(3, T, SFX)                \
+    DEF_XBATCHED(4, T, SFX) DEF_XBATCHED(5, T, SFX) DEF_XBATCHED(6, T, SFX)
+DEF_ALL(double, f64)
+DEF_ALL(float, f32)
+
+#define DEF_VBATCHED(T, SFX)                                                               \
+    void kronmult1_vbatched(int const m1, int const n1, KRONMULT_VB_TAIL(T))               \
+    {                                                                                      \
+        const int m[1] = {m1}, n[1] = {n1};                                                \
+        report("kronmult1_vbatched", kron_vbatched_##SFX(1, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }                                                                                      \
+    void kronmult2_vbatched(int const m1, int const n1, int const m2, int const n2,        \
+                            KRONMULT_VB_TAIL(T))                                           \
+    {                                                                                      \
+        const int m[2] = {m1, m2}, n[2] = {n1, n2};                                        \
+        report("kronmult2_vbatched", kron_vbatched_##SFX(2, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }                                                                                      \
+    void kronmult3_vbatched(int const m1, int const n1, int const m2, int const n2,        \
+                            int const m3, int const n3, KRONMULT_VB_TAIL(T))               \
+    {                                                                                      \
+        const int m[3] = {m1, m2, m3}, n[3] = {n1, n2, n3};                                \
+        report("kronmult3_vbatched", kron_vbatched_##SFX(3, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }                                                                                      \
+    void kronmult4_vbatched(int const m1, int const n1, int const m2, int const n2,        \
+                            int const m3, int const n3, int const m4, int const n4,        \
+                            KRONMULT_VB_TAIL(T))                                           \
+    {                                                                                      \
+        const int m[4] = {m1, m2, m3, m4}, n[4] = {n1, n2, n3, n4};                        \
+        report("kronmult4_vbatched", kron_vbatched_##SFX(4, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }                                                                                      \
+    void kronmult5_vbatched(int const m1, int const n1, int const m2, int const n2,        \
+                            int const m3, int const n3, int const m4, int const n4,        \
+                            int const m5, int const n5, KRONMULT_VB_TAIL(T))               \
+    {                                                                                      \
+        const int m[5] = {m1, m2, m3, m4, m5}, n[5] = {n1, n2, n3, n4, n5};                \
+        report("kronmult5_vbatched", kron_vbatched_##SFX(5, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }                                                                                      \
+    void kronmult6_vbatched(int const m1, int const n1, int const m2, int const n2,        \
+                            int const m3, int const n3, int const m4, int const n4,        \
+                            int const m5, int const n5, int const m6, int const n6,        \
+                            KRONMULT_VB_TAIL(T))                                           \
+    {                                                                                      \
+        const int m[6] = {m1, m2, m3, m4, m5, m6}, n[6] = {n1, n2, n3, n4, n5, n6};        \
+        report("kronmult6_vbatched", kron_vbatched_##SFX(6, m, n, Aarray_, pX_, pY_, W_,   \
+                                                         Wcapcity, batchCount, nullptr));  \
+    }
+DEF_VBATCHED(double, f64)
+DEF_VBATCHED(float, f32)
+
+#define DEF_KGEMM(NAME, T, CNAME)                                                          \
+    void NAME(int const mm, int const nn, int const kk, T const alpha, T *const Aarray_[], \
+              int const ldAarray_[], T *const Barray_[], int const ldBarray_[],            \
+              T const beta, T *const Carray_[], int const ldCarray_[],                     \
+              int const batchCount)                                                        \
+    {                                                                                      \
+        report(#NAME, CNAME(mm, nn, kk, alpha, Aarray_, ldAarray_, Barray_, ldBarray_,     \
+                            beta, Carray_, ldCarray_, batchCount, nullptr));               \
+    }
+DEF_KGEMM(kgemm_nn_batched, double, kron_kgemm_nn_batched_f64)
+DEF_KGEMM(kgemm_nt_batched, double, kron_kgemm_nt_batched_f64)
+DEF_KGEMM(kgemm_nn_batched, float, kron_kgemm_nn_batched_f32)
+DEF_KGEMM(kgemm_nt_batched, float, kron_kgemm_nt_batched_f32)

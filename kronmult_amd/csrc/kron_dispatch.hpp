@@ -1,0 +1,20 @@
+// kron_dispatch.hpp -- entry points of the separately compiled fast-kernel
+// translation units (kron_fast_*.hip), one per (element type, layout).
+#pragma once
+
+#include "kron_device.hpp"
+
+namespace kron {
+
+// Batched layout (Y private per entry).  trans selects the adjoint factors.
+kron_status fast_batched_f64(int D, int N, const Entries<double, Layout::Batched> &e,
+                             bool trans, int batch, hipStream_t s);
+kron_status fast_batched_f32(int D, int N, const Entries<float, Layout::Batched> &e,
+                             bool trans, int batch, hipStream_t s);
+// Pointer layout (Y may be shared -> atomics), leading dimension lda.
+kron_status fast_ptr_f64(int D, int N, const Entries<double, Layout::Ptr> &e, int lda,
+                         int batch, hipStream_t s);
+kron_status fast_ptr_f32(int D, int N, const Entries<float, Layout::Ptr> &e, int lda,
+                         int batch, hipStream_t s);
+
+}  // namespace kron

@@ -1,0 +1,15 @@
+// Fast fused kernels, double, batched layout (kronmult{1..6}_batched).
+#include "kron_fast.hpp"
+#include "kron_dispatch.hpp"
+
+namespace kron {
+
+kron_status fast_batched_f64(int D, int N, const Entries<double, Layout::Batched> &e,
+                             bool trans, int batch, hipStream_t s)
+{
+    if (trans)
+        return launch_fast<double, double, Layout::Batched, false, true>(D, N, e, N, batch, s);
+    return launch_fast<double, double, Layout::Batched, false, false>(D, N, e, N, batch, s);
+}
+
+}  // namespace kron

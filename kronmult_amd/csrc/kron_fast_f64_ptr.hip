@@ -1,0 +1,13 @@
+// Fast fused kernels, double, pointer layout (kronmult{1..6}_xbatched, square vbatched).
+#include "kron_fast.hpp"
+#include "kron_dispatch.hpp"
+
+namespace kron {
+
+kron_status fast_ptr_f64(int D, int N, const Entries<double, Layout::Ptr> &e, int lda,
+                         int batch, hipStream_t s)
+{
+    return launch_fast<double, double, Layout::Ptr, true, false>(D, N, e, lda, batch, s);
+}
+
+}  // namespace kron

@@ -1,0 +1,365 @@
+// kron_generic.hip -- runtime-shape kernel and the forward dispatcher.
+//
+// The generic kernel covers every shape the compile-time fast kernel does
+// not: rectangular factors (vbatched, kronmult_vbatched.hpp:63-68), n > 8,
+// and entries whose n^d does not fit in LDS.  One 256-thread workgroup owns an
+// entry at a time.
+//
+//   single pass (h == 0): stage X in LDS, contract modes 1..d with LDS
+//       ping-pong buffers, add the result into Y.
+//   two passes  (0 < h < d), when the entry is larger than LDS:
+//       pass 1: for chunks of consecutive low indices, contract the slow modes
+//               1..h in LDS and store the chunk to the entry's scratch vector;
+//       pass 2: for chunks of slow-index slabs, contract the fast modes
+//               h+1..d in LDS and add the contiguous slabs into Y.
+//   The scratch holds m1..mh x n(h+1)..nd elements, one of the reference's
+//   own W sizes (kronmult_vbatched.hpp:129-153), so the reference's workspace
+//   rules suffice.  HBM traffic is X + 2 * scratch + Y RMW, against the
+//   reference's 2*d passes over the whole vector.
+//
+// Inside a pass every output element is one work item: n_k LDS reads and n_k
+// FMAs, consecutive threads on consecutive fast indices.
+#include <algorithm>
+
+#include "kron_dispatch.hpp"
+#include "kron_fast.hpp"
+
+namespace kron {
+
+struct GenShape {
+    int d;
+    int h;                  // split point (0 = single pass)
+    int m[kMaxDim];         // output extent of mode k (0-based, slowest first)
+    int n[kMaxDim];         // input extent of mode k
+    int ld[kMaxDim];        // leading dimension of factor k as stored
+    int chunk1;             // low indices per pass-1 chunk
+    int chunk2;             // slabs per pass-2 chunk
+    int lds_elems;          // elements per LDS ping-pong buffer
+    long long nlo_in;       // prod_{k>=h} n_k
+    long long nhi_in;       // prod_{k<h}  n_k
+    long long nhi_out;      // prod_{k<h}  m_k
+    long long nlo_out;      // prod_{k>=h} m_k
+};
+
+template <typename T>
+struct ScratchDesc {
+    Scratch kind;
+    T *W;                 // BatchedW base
+    long long wstride;    // BatchedW elements per entry
+    T *const *pW;         // PtrSlots / PtrSlot0
+    T *Wbase;             // Linear
+    long long slot_stride;
+    __device__ __forceinline__ T *get(int slot, int b) const
+    {
+        switch (kind) {
+        case Scratch::BatchedW: return W + (long long)b * wstride;
+        case Scratch::PtrSlots: return pW[2 * slot];
+        case Scratch::PtrSlot0: return pW[0];
+        case Scratch::Linear: return Wbase + (long long)slot * slot_stride;
+        default: return nullptr;
+        }
+    }
+};
+
+// dst(o, i, ii) = sum_j A(i, j) * src(o, j, ii) for mode q of a tensor with
+// extents ext[0..R), ext[q] = n on input, m on output.  Output index == item.
+template <typename T, typename Acc, bool TRANS>
+__device__ __forceinline__ void contract_mode(const T *__restrict__ src, T *__restrict__ dst,
+                                              const int *ext, int R, int q, int m,
+                                              const T *__restrict__ Ap, int ld)
+{
+    int inner = 1;
+    for (int r = q + 1; r < R; ++r) inner *= ext[r];
+    int outer = 1;
+    for (int r = 0; r < q; ++r) outer *= ext[r];
+    const int n = ext[q];
+    const int total = outer * m * inner;
+    for (int w = threadIdx.x; w < total; w += blockDim.x) {
+        const int ii = w % inner;
+        const int rest = w / inner;
+        const int i = rest % m;
+        const int o = rest / m;
+        const T *s = src + (o * n) * inner + ii;
+        Acc acc = 0;
+        for (int j = 0; j < n; ++j) {
+            const Acc aij = TRANS ? Acc(Ap[j + i * ld]) : Acc(Ap[i + j * ld]);
+            acc = __builtin_fma(aij, Acc(s[j * inner]), acc);
+        }
+        dst[w] = T(acc);
+    }
+}
+
+template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS>
+__global__ void __launch_bounds__(256)
+kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int batch)
+{
+    extern __shared__ __align__(16) unsigned char kron_smem[];
+    T *const buf0 = reinterpret_cast<T *>(kron_smem);
+    T *const buf1 = buf0 + sh.lds_elems;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int d = sh.d;
+    const int h = sh.h;
+
+    for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+        const T *__restrict__ x = ent.x(b);
+        T *y = ent.y(b);
+        T *S = (h > 0) ? scr.get(blockIdx.x, b) : nullptr;
+        int ext[kMaxDim + 1];
+
+        if (h > 0) {
+            // pass 1: slow modes 0..h-1 over chunks of low indices
+            for (long long c0 = 0; c0 < sh.nlo_in; c0 += sh.chunk1) {
+                const int C = (int)min((long long)sh.chunk1, sh.nlo_in - c0);
+                const int nin = (int)sh.nhi_in * C;
+                for (int w = tid; w < nin; w += nthr) {
+                    const int hi = w / C, c = w % C;
+                    buf0[w] = x[(long long)hi * sh.nlo_in + c0 + c];
+                }
+                __syncthreads();
+                for (int k = 0; k < h; ++k) ext[k] = sh.n[k];
+                ext[h] = C;
+                T *src = buf0, *dst = buf1;
+                for (int q = 0; q < h; ++q) {
+                    contract_mode<T, Acc, TRANS>(src, dst, ext, h + 1, q, sh.m[q],
+                                                 ent.factor(b, q), sh.ld[q]);
+                    ext[q] = sh.m[q];
+                    __syncthreads();
+                    T *t = src; src = dst; dst = t;
+                }
+                const int nout = (int)sh.nhi_out * C;
+                for (int w = tid; w < nout; w += nthr) {
+                    const int hi = w / C, c = w % C;
+                    S[(long long)hi * sh.nlo_in + c0 + c] = src[w];
+                }
+                __syncthreads();
+            }
+        }
+
+        // pass 2 (or the only pass): fast modes h..d-1 over slabs
+        const T *__restrict__ src2 = (h > 0) ? S : x;
+        for (long long s0 = 0; s0 < sh.nhi_out; s0 += sh.chunk2) {
+            const int C2 = (int)min((long long)sh.chunk2, sh.nhi_out - s0);
+            const int nin = C2 * (int)sh.nlo_in;
+            const T *srcg = src2 + s0 * sh.nlo_in;
+            for (int w = tid; w < nin; w += nthr) buf0[w] = srcg[w];
+            __syncthreads();
+            ext[0] = C2;
+            for (int k = h; k < d; ++k) ext[1 + k - h] = sh.n[k];
+            const int R = 1 + d - h;
+            T *src = buf0, *dst = buf1;
+            for (int k = h; k < d; ++k) {
+                const int q = 1 + k - h;
+                contract_mode<T, Acc, TRANS>(src, dst, ext, R, q, sh.m[k], ent.factor(b, k),
+                                             sh.ld[k]);
+                ext[q] = sh.m[k];
+                __syncthreads();
+                T *t = src; src = dst; dst = t;
+            }
+            const int nout = C2 * (int)sh.nlo_out;
+            const long long ybase = s0 * sh.nlo_out;
+            for (int w = tid; w < nout; w += nthr)
+                y_accumulate<SHARED, T, Acc>(y, ybase + w, Acc(src[w]));
+            __syncthreads();
+        }
+    }
+}
+
+// Choose the split and chunking for an LDS budget of `budget` elements per
+// ping-pong buffer.  Returns false when no split fits.
+static bool plan_generic(GenShape &sh, long long budget)
+{
+    const int d = sh.d;
+    // largest intermediate of contracting modes [a, b) in ascending order on a
+    // tensor whose other modes keep their input extents (mixed m-prefix).
+    auto peak = [&](int a, int b) {
+        long long best = 0;
+        for (int k = a - 1; k < b; ++k) {
+            long long s = 1;
+            for (int l = a; l < b; ++l) s *= (l <= k) ? sh.m[l] : sh.n[l];
+            best = std::max(best, s);
+        }
+        return best;
+    };
+    long long prod_n = 1;
+    for (int k = 0; k < d; ++k) prod_n *= sh.n[k];
+    const long long single = peak(0, d);
+    if (single <= budget) {
+        sh.h = 0;
+        sh.chunk1 = 0;
+        sh.chunk2 = 1;
+        sh.lds_elems = (int)single;
+        sh.nhi_in = sh.nhi_out = 1;
+        sh.nlo_in = prod_n;
+        long long mo = 1;
+        for (int k = 0; k < d; ++k) mo *= sh.m[k];
+        sh.nlo_out = mo;
+        return true;
+    }
+    bool found = false;
+    long long best_iters = 0;
+    for (int h = 1; h < d; ++h) {
+        long long nhi_in = 1, nhi_out = 1, nlo_in = 1, nlo_out = 1;
+        for (int k = 0; k < h; ++k) { nhi_in *= sh.n[k]; nhi_out *= sh.m[k]; }
+        for (int k = h; k < d; ++k) { nlo_in *= sh.n[k]; nlo_out *= sh.m[k]; }
+        const long long per1 = peak(0, h), per2 = peak(h, d);
+        long long c1 = std::min(nlo_in, budget / per1);
+        long long c2 = std::min(nhi_out, budget / per2);
+        if (c1 < 1 || c2 < 1) continue;
+        const long long iters = (nlo_in + c1 - 1) / c1 + (nhi_out + c2 - 1) / c2;
+        if (!found || iters < best_iters) {
+            found = true;
+            best_iters = iters;
+            sh.h = h;
+            sh.chunk1 = (int)c1;
+            sh.chunk2 = (int)c2;
+            sh.lds_elems = (int)std::max(per1 * c1, per2 * c2);
+            sh.nhi_in = nhi_in;
+            sh.nhi_out = nhi_out;
+            sh.nlo_in = nlo_in;
+            sh.nlo_out = nlo_out;
+        }
+    }
+    return found;
+}
+
+template <typename T>
+static bool make_generic_shape(const KronArgs<T> &a, GenShape &sh)
+{
+    sh.d = a.d;
+    for (int k = 0; k < kMaxDim; ++k) {
+        // the operator applied is A_k (m_k x n_k) or A_k^T (n_k x m_k)
+        sh.m[k] = a.trans ? a.n[k] : a.m[k];
+        sh.n[k] = a.trans ? a.m[k] : a.n[k];
+        sh.ld[k] = (a.lda > 0) ? a.lda : a.m[k];
+    }
+    const long long budget = (long long)(max_lds_bytes() / 2) / (long long)sizeof(T);
+    return plan_generic(sh, budget);
+}
+
+template <typename T, Layout LAY, bool SHARED, bool TRANS>
+static kron_status launch_generic_t(const Entries<T, LAY> &ent, const GenShape &sh,
+                                    const ScratchDesc<T> &scr, int grid, int batch,
+                                    hipStream_t stream)
+{
+    auto kern = kron_generic_kernel<T, T, LAY, SHARED, TRANS>;
+    const size_t lds = 2 * (size_t)sh.lds_elems * sizeof(T);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, stream, ent, sh, scr, batch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
+    return KRON_OK;
+}
+
+template <typename T>
+size_t kron_scratch_elems(const KronArgs<T> &a)
+{
+    bool square = true;
+    for (int k = 0; k < a.d; ++k) square = square && (a.m[k] == a.n[k]) && a.m[k] == a.m[0];
+    if (square && a.n[0] <= fast_max_n<T>(a.d)) {
+        long long e = 1;
+        for (int k = 0; k < a.d; ++k) e *= a.n[0];
+        if (a.d == 1 || (long long)sizeof(T) * e <= max_lds_bytes()) return 0;
+    }
+    GenShape sh{};
+    if (!make_generic_shape(a, sh)) return (size_t)-1;
+    return sh.h == 0 ? 0 : (size_t)(sh.nhi_out * sh.nlo_in);
+}
+
+static kron_status fast_dispatch(const KronArgs<double> &a, int N, hipStream_t s)
+{
+    if (a.layout == Layout::Batched) {
+        Entries<double, Layout::Batched> e{a.A, a.X, a.Y, a.d, (long long)N * N, 0, 0};
+        long long E = 1;
+        for (int k = 0; k < a.d; ++k) E *= N;
+        e.sx = e.sy = E;
+        return fast_batched_f64(a.d, N, e, a.trans, a.batch, s);
+    }
+    Entries<double, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
+    return fast_ptr_f64(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s);
+}
+
+static kron_status fast_dispatch(const KronArgs<float> &a, int N, hipStream_t s)
+{
+    if (a.layout == Layout::Batched) {
+        Entries<float, Layout::Batched> e{a.A, a.X, a.Y, a.d, (long long)N * N, 0, 0};
+        long long E = 1;
+        for (int k = 0; k < a.d; ++k) E *= N;
+        e.sx = e.sy = E;
+        return fast_batched_f32(a.d, N, e, a.trans, a.batch, s);
+    }
+    Entries<float, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
+    return fast_ptr_f32(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s);
+}
+
+template <typename T>
+kron_status launch_kron(const KronArgs<T> &a)
+{
+    if (a.batch == 0) return KRON_OK;
+    bool square = true;
+    for (int k = 0; k < a.d; ++k) square = square && (a.m[k] == a.n[k]) && a.m[k] == a.m[0];
+    // Fast path: square, compile-time (d, n), fits in LDS.  Transposed factors
+    // are only compiled for the batched layout (the adjoint).
+    if (square && (!a.trans || a.layout == Layout::Batched)) {
+        kron_status st = fast_dispatch(a, a.n[0], a.stream);
+        if (st != KRON_ENOTSUP) return st;
+    }
+    GenShape sh{};
+    if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
+    ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
+    int grid = a.batch;
+    if (sh.h > 0) {
+        scr.kind = a.scratch;
+        switch (a.scratch) {
+        case Scratch::BatchedW:
+            if (!a.W) return KRON_EWORKSPACE;
+            scr.W = a.W;
+            scr.wstride = sh.nlo_in * sh.nhi_in;  // batched is square: sizeX
+            break;
+        case Scratch::PtrSlots:
+        case Scratch::PtrSlot0:
+            if (!a.pW) return KRON_EWORKSPACE;
+            scr.pW = a.pW;
+            grid = std::min(a.batch, a.scratch == Scratch::PtrSlot0 ? 1 : a.nslots);
+            break;
+        case Scratch::Linear:
+            if (!a.Wbase || a.nslots < 1) return KRON_EWORKSPACE;
+            scr.Wbase = a.Wbase;
+            scr.slot_stride = (long long)a.slot_stride;
+            grid = std::min(a.batch, a.nslots);
+            break;
+        default:
+            return KRON_EWORKSPACE;
+        }
+        if (grid < 1) return KRON_EWORKSPACE;
+    }
+    if (a.layout == Layout::Batched) {
+        Entries<T, Layout::Batched> e{a.A, a.X, a.Y, a.d, 0, 0, 0};
+        e.fstride = (long long)a.m[0] * a.n[0];
+        long long sx = 1, sy = 1;
+        for (int k = 0; k < a.d; ++k) { sx *= a.n[k]; sy *= a.m[k]; }
+        e.sx = sx;
+        e.sy = sy;
+        if (a.trans) {
+            std::swap(e.sx, e.sy);
+            return launch_generic_t<T, Layout::Batched, false, true>(e, sh, scr, grid, a.batch,
+                                                                    a.stream);
+        }
+        return launch_generic_t<T, Layout::Batched, false, false>(e, sh, scr, grid, a.batch,
+                                                                 a.stream);
+    }
+    Entries<T, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
+    if (a.trans) return KRON_ENOTSUP;
+    return launch_generic_t<T, Layout::Ptr, true, false>(e, sh, scr, grid, a.batch, a.stream);
+}
+
+template kron_status launch_kron<double>(const KronArgs<double> &);
+template kron_status launch_kron<float>(const KronArgs<float> &);
+template size_t kron_scratch_elems<double>(const KronArgs<double> &);
+template size_t kron_scratch_elems<float>(const KronArgs<float> &);
+
+}  // namespace kron

@@ -1,0 +1,79 @@
+// kron_internal.hpp -- host-side interface between the C ABI (capi.cpp) and
+// the HIP kernels (kron_*.hip).  Not installed; no torch types anywhere.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/kronmult_amd.h"
+
+namespace kron {
+
+constexpr int kMaxDim = 6;
+
+// Where a call's entries live.  Batched: contiguous arrays (A (n,n,d,batch),
+// X/Y/W (n^d,batch)).  Ptr: per-entry pointer arrays in device memory
+// (Aarray[k + b*d], pX[b], pY[b]); Y targets may coincide.
+enum class Layout : int { Batched = 0, Ptr = 1 };
+
+// Where the per-entry scratch of a shape that does not fit in LDS comes from
+// (mirrors the reference's workspace rules).
+enum class Scratch : int {
+    None = 0,      // shape fits on chip; no scratch is touched
+    BatchedW = 1,  // W + b*sizeX (batched layout: W is (n^d, batch))
+    PtrSlots = 2,  // pW[2*slot] (xbatched, batch > 1; kronmult_xbatched.hpp:125-126)
+    PtrSlot0 = 3,  // pW[0]      (xbatched, batch == 1)
+    Linear = 4     // Wbase + slot*slot_stride (vbatched; kronmult_vbatched.hpp:184-185)
+};
+
+template <typename T>
+struct KronArgs {
+    int d = 0;
+    int m[kMaxDim] = {1, 1, 1, 1, 1, 1};  // output extents (rows of A_k)
+    int n[kMaxDim] = {1, 1, 1, 1, 1, 1};  // input extents (cols of A_k)
+    int lda = 0;          // leading dim of every factor; 0 -> ld_k = m_k
+    bool trans = false;   // apply A_k^T (adjoint); factors are then n_k x m_k
+    Layout layout = Layout::Batched;
+    // batched layout
+    const T *A = nullptr;
+    const T *X = nullptr;
+    T *Y = nullptr;
+    T *W = nullptr;
+    // pointer layout
+    const T *const *pA = nullptr;
+    const T *const *pX = nullptr;
+    T *const *pY = nullptr;
+    T *const *pW = nullptr;
+    T *Wbase = nullptr;
+    size_t slot_stride = 0;  // elements between Linear scratch slots
+    int nslots = 0;          // number of scratch slots available
+    Scratch scratch = Scratch::None;
+    bool y_shared = true;    // entries may share Y -> atomic adds
+    int batch = 0;
+    hipStream_t stream = nullptr;
+};
+
+// Launch the forward kernel family.  Returns KRON_OK, KRON_ENOTSUP when no
+// kernel covers the shape, KRON_EWORKSPACE when scratch is needed but absent,
+// or KRON_EHIP on a launch failure.
+template <typename T>
+kron_status launch_kron(const KronArgs<T> &a);
+
+// Scratch elements per entry that launch_kron needs for this shape (0 when the
+// shape runs entirely on chip).
+template <typename T>
+size_t kron_scratch_elems(const KronArgs<T> &a);
+
+// Batched kgemm (kgemm_nn_batched.hpp:8-79 / kgemm_nt_batched.hpp:8-75).
+template <typename T>
+kron_status launch_kgemm(bool transB, int mm, int nn, int kk, T alpha,
+                         const T *const *A, const int *ldA, const T *const *B,
+                         const int *ldB, T beta, T *const *C, const int *ldC,
+                         int batch, hipStream_t stream);
+
+// Per-thread record of the last HIP error (kron_last_hip_error()).
+void set_last_hip_error(int code);
+int max_lds_bytes();
+
+}  // namespace kron

@@ -1,0 +1,339 @@
+"""GPU parity: the HIP library through its C ABI against the oracle.
+
+Bars: FP64 within 1e-12 (the reference tests' relative measure,
+|a-b| / (1 + max(|a|,|b|))); integer-valued inputs bit-exact, which pins
+pointer and index handling regardless of atomic order; FP32 within the
+reference's FP32 tolerance 1e-5 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import kronmult_amd as K  # noqa: E402
+from gpu_util import DEV, dev, host, ints, ptrs  # noqa: E402
+
+O = oracle.oracle_lib()
+F64_TOL = 1e-12
+F32_TOL = 1e-5
+
+
+def _oracle_batched(d, n, A, X, Y0):
+    Y, Xc, W = Y0.astype(np.float64), X.astype(np.float64), np.zeros(X.shape)
+    oracle.batched(O, d, n, np.ascontiguousarray(A, dtype=np.float64), Xc, Y, W, X.shape[0])
+    return Y
+
+
+def _gpu_batched(d, n, A, X, Y0, with_w=True):
+    Ad, Xd, Yd = dev(A), dev(X), dev(Y0)
+    Wd = torch.empty_like(Xd) if with_w else None
+    X_before = Xd.clone()
+    K.kronmult_batched(d, n, Ad, Xd, Yd, Wd, X.shape[0])
+    out = host(Yd)
+    assert torch.equal(Xd, X_before), "X must not be written"
+    return out
+
+
+FAST_SHAPES = [(d, n) for d in range(1, 7) for n in range(1, 9) if n ** d <= 16807]
+
+
+@pytest.mark.parametrize("d,n", FAST_SHAPES)
+def test_batched_f64_uniform(d, n):
+    batch = 7 if n ** d <= 4096 else 3
+    E = n ** d
+    A = H.uniform(batch * d * n * n, d * 100 + n).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, d * 100 + n + 50).reshape(batch, E)
+    Y0 = H.uniform(batch * E, 3).reshape(batch, E)
+    assert H.rel_err(_gpu_batched(d, n, A, X, Y0), _oracle_batched(d, n, A, X, Y0)) < F64_TOL
+
+
+@pytest.mark.parametrize("d", range(1, 7))
+def test_batched_reference_sweep(d):
+    """test_kronmult6_batched.cpp:511-567: batch {1,16,128} x n {1,2,3,4},
+    closed-form inputs."""
+    for batch in (1, 16, 128):
+        for n in (1, 2, 3, 4):
+            E = n ** d
+            A, X = H.ref_a(n, d, batch), H.ref_x(E, batch)
+            Y0 = np.zeros((batch, E))
+            got = _gpu_batched(d, n, A, X, Y0)
+            want = _oracle_batched(d, n, A, X, Y0)
+            assert np.max(np.abs(got - want)) <= 1e-6
+            assert H.rel_err(got, want) < F64_TOL, (batch, n)
+
+
+@pytest.mark.parametrize("d,n", [(6, 4), (5, 5), (3, 8), (4, 6), (2, 7), (1, 8)])
+def test_batched_integer_bit_exact(d, n):
+    batch, E = 33, n ** d
+    A = H.integers(batch * d * n * n, 11, -2, 2).reshape(batch, d, n * n)
+    X = H.integers(batch * E, 12, -8, 8).reshape(batch, E)
+    Y0 = H.integers(batch * E, 13, -100, 100).reshape(batch, E)
+    assert np.array_equal(_gpu_batched(d, n, A, X, Y0), _oracle_batched(d, n, A, X, Y0))
+
+
+@pytest.mark.parametrize("d,n", [(6, 4), (4, 5), (3, 8), (2, 3), (1, 4)])
+def test_batched_f32(d, n):
+    batch, E = 9, n ** d
+    A = H.uniform(batch * d * n * n, 21).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, 22).reshape(batch, E)
+    Y0 = np.zeros((batch, E))
+    got = _gpu_batched(d, n, A.astype(np.float32), X.astype(np.float32), Y0.astype(np.float32))
+    want = _oracle_batched(d, n, A.astype(np.float32), X.astype(np.float32), Y0)
+    assert H.rel_err(got, want) < F32_TOL
+
+
+# generic kernel: n > 8 single pass, and entries larger than LDS (two passes
+# through W)
+@pytest.mark.parametrize("d,n,batch", [(2, 20, 5), (3, 12, 3), (6, 6, 3), (5, 8, 2),
+                                       (3, 32, 2), (4, 16, 2), (1, 40, 4)])
+def test_batched_generic_shapes(d, n, batch):
+    E = n ** d
+    A = H.uniform(batch * d * n * n, 31 + n).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, 32 + n).reshape(batch, E)
+    Y0 = H.uniform(batch * E, 33).reshape(batch, E)
+    assert H.rel_err(_gpu_batched(d, n, A, X, Y0), _oracle_batched(d, n, A, X, Y0)) < F64_TOL
+
+
+def test_batched_errors():
+    A = torch.zeros(16, dtype=torch.float64, device=DEV)
+    with pytest.raises(K.KronError) as e:
+        K.kronmult_batched(7, 2, A, A, A, A, 1)
+    assert e.value.status == K.KRON_EINVAL
+    with pytest.raises(K.KronError):
+        K.kronmult_batched(2, 0, A, A, A, A, 1)
+    # 6^6 needs the W scratch
+    with pytest.raises(K.KronError) as e:
+        K.kronmult_batched(6, 6, A, A, A, None, 1)
+    assert e.value.status == K.KRON_EWORKSPACE
+    K.kronmult_batched(3, 2, A, A, A, None, 0)  # empty batch is a no-op
+
+
+# --- xbatched -----------------------------------------------------------------
+
+def _xbatched_case(d, n, batch, nx, ny, lda, seed, kind="uniform", dtype=np.float64):
+    E = n ** d
+    gen = (lambda c, s: H.uniform(c, s)) if kind == "uniform" else (
+        lambda c, s: H.integers(c, s, -2, 2))
+    A = gen(batch * d * n * n, H.sub_seed(seed, 1)).reshape(batch * d, n, n)
+    A_pool = np.full((batch * d, n, lda), np.nan)
+    A_pool[:, :, :n] = A
+    A_pool = A_pool.reshape(batch * d, lda * n).astype(dtype)
+    xs = H.uniform(nx * E, H.sub_seed(seed, 2)) if kind == "uniform" else H.integers(
+        nx * E, H.sub_seed(seed, 2), -8, 8)
+    X_pool = xs.reshape(nx, E).astype(dtype)
+    x_map = (H.splitmix64(batch, H.sub_seed(seed, 3)) % np.uint64(nx)).astype(np.int64)
+    y_map = (H.splitmix64(batch, H.sub_seed(seed, 4)) % np.uint64(ny)).astype(np.int64)
+    return A_pool, X_pool, x_map, y_map
+
+
+def _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0_pool, y_map):
+    batch = len(x_map)
+    Y = Y0_pool.astype(np.float64)
+    W = np.zeros((max(2, batch), X_pool.shape[1]))
+    oracle.xbatched(O, d, n, A_pool.astype(np.float64), np.arange(batch * d).reshape(batch, d),
+                    lda, X_pool.astype(np.float64), x_map, Y, y_map, W, np.arange(W.shape[0]),
+                    batch)
+    return Y
+
+
+def _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0_pool, y_map, subbatch=0):
+    batch = len(x_map)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0_pool)
+    nslot = max(2, batch)
+    Wd = torch.full((nslot, X_pool.shape[1]), float("nan"), dtype=Xd.dtype, device=DEV)
+    pA = ptrs(Ad, np.arange(batch * d))
+    pX, pY, pW = ptrs(Xd, x_map), ptrs(Yd, y_map), ptrs(Wd, np.arange(nslot))
+    X_before = Xd.clone()
+    K.kronmult_xbatched(d, n, pA, lda, pX, pY, pW, batch, subbatch, dtype=Xd.dtype)
+    out = host(Yd)
+    assert torch.equal(Xd, X_before), "X must not be written"
+    return out
+
+
+@pytest.mark.parametrize("d", range(1, 7))
+def test_xbatched_reference_sweep(d):
+    """test_kronmult6_xbatched.cpp: batch {1,16,128} x n {1,2,3,4}, lda = n+3,
+    every entry adds into Y(:,1), closed-form inputs; tol 3e-7 there, 1e-12 here."""
+    for batch in (1, 16, 128):
+        for n in (1, 2, 3, 4):
+            E, lda = n ** d, n + 3
+            A_pool = H.ref_a(n, d, batch, lda=lda).reshape(batch * d, lda * n)
+            X_pool = H.ref_x(E, batch)
+            Y0 = np.zeros((1, E))
+            y_map = np.zeros(batch, dtype=np.int64)
+            got = _gpu_xbatched(d, n, lda, A_pool, X_pool, np.arange(batch), Y0, y_map)
+            want = _oracle_xbatched(d, n, lda, A_pool, X_pool, np.arange(batch), Y0, y_map)
+            assert H.rel_err(got, want) < F64_TOL, (batch, n)
+
+
+@pytest.mark.parametrize("d,n", [(6, 4), (5, 3), (4, 5), (3, 8), (2, 6), (1, 7), (6, 5)])
+def test_xbatched_shared_y_integer_bit_exact(d, n):
+    batch, lda = 300, n + 2
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 40, 9, lda, 5, kind="int")
+    Y0 = np.zeros((9, n ** d))
+    got = _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("d,n", [(6, 4), (4, 7), (3, 12), (6, 6)])
+def test_xbatched_uniform(d, n):
+    batch, lda = 64 if n ** d < 10000 else 6, n
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 16, 5, lda, 6)
+    Y0 = H.uniform(5 * n ** d, 8).reshape(5, -1)
+    got = _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map, subbatch=3)
+    want = _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert H.rel_err(got, want) < F64_TOL
+
+
+def test_xbatched_f32():
+    d, n, batch = 6, 4, 50
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 10, 4, n, 9)
+    Y0 = np.zeros((4, n ** d))
+    got = _gpu_xbatched(d, n, n, A_pool.astype(np.float32), X_pool.astype(np.float32), x_map,
+                        Y0.astype(np.float32), y_map)
+    want = _oracle_xbatched(d, n, n, A_pool.astype(np.float32).astype(np.float64),
+                            X_pool.astype(np.float32).astype(np.float64), x_map, Y0, y_map)
+    assert H.rel_err(got, want) < F32_TOL
+
+
+def test_xbatched_headline_shape_integer():
+    """d=6, n=4, 65,536 entries, X pool 8,192, Y pool 2,048 (about 32 writers
+    per Y): bit-exact with integer inputs at the full size."""
+    d, n, batch = 6, 4, 65536
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 8192, 2048, n, 77, kind="int")
+    Y0 = np.zeros((2048, n ** d))
+    got = _gpu_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+# --- vbatched -----------------------------------------------------------------
+
+def _vb_case(m, n, batch, seed, ny):
+    d = len(m)
+    A_pool = np.zeros((batch * d, max(mk * nk for mk, nk in zip(m, n))))
+    for b in range(batch):
+        for k in range(d):
+            A_pool[b * d + k, : m[k] * n[k]] = H.uniform(m[k] * n[k], seed * 1000 + b * 10 + k)
+    sx = int(np.prod(n))
+    X_pool = H.uniform(batch * sx, seed + 5).reshape(batch, sx)
+    y_map = np.arange(batch) % ny
+    return A_pool, X_pool, y_map
+
+
+def _sizes(m, n):
+    d = len(m)
+    sx = int(np.prod(n))
+    sw = max(int(np.prod(m[:k + 1])) * int(np.prod(n[k + 1:])) for k in range(d))
+    return sx, max(sx, sw)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_vbatched_rectangular(seed):
+    rng = np.random.default_rng(seed + 100)
+    d = int(rng.integers(1, 7))
+    hi = 4 if d > 3 else 9
+    m = [int(v) for v in rng.integers(1, hi, d)]
+    n = [int(v) for v in rng.integers(1, hi, d)]
+    batch, ny = 10, 3
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, seed, ny)
+    sy = int(np.prod(m))
+    Y0 = np.zeros((ny, sy))
+    _, sxw = _sizes(m, n)
+    Wcap = 3 * 2 * sxw * 8
+    Yw = Y0.copy()
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * d).reshape(batch, d), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * d)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
+
+
+def test_vbatched_large_two_pass_and_workspace_rule():
+    m, n = [7, 6, 9, 8, 7], [8, 7, 6, 7, 8]
+    batch = 3
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, 3, 2)
+    sy = int(np.prod(m))
+    _, sxw = _sizes(m, n)
+    Y0 = np.zeros((2, sy))
+    Yw = Y0.copy()
+    Wcap = 2 * sxw * 8  # exactly one slot
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * 5).reshape(batch, 5), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    args = (ptrs(Ad, np.arange(batch * 5)), ptrs(Xd, np.arange(batch)), ptrs(Yd, y_map), Wd)
+    K.kronmult_vbatched(m, n, *args, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL
+    with pytest.raises(K.KronError) as e:
+        K.kronmult_vbatched(m, n, *args, Wcap - 8, batch)
+    assert e.value.status == K.KRON_EWORKSPACE
+
+
+def test_vbatched_square_headline_shape():
+    m = n = [4] * 6
+    batch = 100
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, 4, 7)
+    Y0 = np.zeros((7, 4096))
+    Yw = Y0.copy()
+    Wcap = 2 * 4096 * 8
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * 6).reshape(batch, 6), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    K.kronmult6_vbatched(4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, ptrs(Ad, np.arange(batch * 6)),
+                         ptrs(Xd, np.arange(batch)), ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL
+
+
+# --- kgemm --------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("beta", [0.0, 1.0, 1.2])
+def test_kgemm(trans, beta, dtype):
+    """Subset of the reference sweep mm,nn,kk in 1..65 step 7 (test_kgemm_nn_batched.cpp
+    :534-548) and its perf shapes; tolerance 1e-9 (f64) / 5e-4 (f32) there."""
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    for mm, nn, kk in [(1, 1, 1), (8, 15, 22), (64, 57, 43), (29, 1, 65), (4, 1024, 4),
+                       (512, 8, 8)]:
+        batch = 5
+        A = H.uniform(batch * mm * kk, mm * 3 + kk, dtype).reshape(batch, -1)
+        B = H.uniform(batch * nn * kk, nn + 5, dtype).reshape(batch, -1)
+        C0 = (np.full((batch, mm * nn), np.nan) if beta == 0.0 else
+              H.uniform(batch * mm * nn, 6).reshape(batch, -1)).astype(dtype)
+        ldb = kk if trans == "N" else nn
+        Cw = C0.copy()
+        oracle.kgemm(O, trans, mm, nn, kk, 1.3, A, [mm] * batch, B, [ldb] * batch, beta, Cw,
+                     [mm] * batch, batch)
+        Ad, Bd, Cd = dev(A), dev(B), dev(C0)
+        f = K.kgemm_nn_batched if trans == "N" else K.kgemm_nt_batched
+        f(mm, nn, kk, 1.3, ptrs(Ad, np.arange(batch)), ints([mm] * batch),
+          ptrs(Bd, np.arange(batch)), ints([ldb] * batch), beta, ptrs(Cd, np.arange(batch)),
+          ints([mm] * batch), batch, dtype=Ad.dtype)
+        got = host(Cd)
+        assert np.all(np.isfinite(got))
+        assert H.rel_err(got, Cw) < tol, (mm, nn, kk)
+
+
+def test_kgemm_shared_c_atomic():
+    """beta == 1 with every entry naming the same C: atomic accumulation."""
+    mm, nn, kk, batch = 16, 16, 4, 200
+    A = H.integers(batch * mm * kk, 1, -3, 3).reshape(batch, -1)
+    B = H.integers(batch * kk * nn, 2, -3, 3).reshape(batch, -1)
+    C0 = np.zeros((1, mm * nn))
+    want = np.zeros((nn, mm))
+    for b in range(batch):
+        want += (A[b].reshape(kk, mm).T @ B[b].reshape(nn, kk).T).T
+    Ad, Bd, Cd = dev(A), dev(B), dev(C0)
+    K.kgemm_nn_batched(mm, nn, kk, 1.0, ptrs(Ad, np.arange(batch)), ints([mm] * batch),
+                       ptrs(Bd, np.arange(batch)), ints([kk] * batch), 1.0,
+                       ptrs(Cd, np.zeros(batch)), ints([mm] * batch), batch)
+    assert np.array_equal(host(Cd)[0], want.reshape(-1))
