@@ -1,0 +1,254 @@
+"""Benchmark of the batched Kronecker matvec on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Default workload (the headline): kronmult6_xbatched, FP64, d=6, n=4,
+65,536 entries, distinct A factors per entry (uniform [-1,1]), X pointers
+drawn uniformly from a pool of 8,192 vectors of 4,096, Y pointers drawn
+uniformly from a pool of 2,048 vectors (about 32 atomic writers per Y), fixed
+seed.  A step is one call over the whole batch, inputs resident in HBM.
+
+Printed: one JSON line with entries/s, the roofline of the dominant kernel
+(algorithmic bytes per launch / its average duration from HIP events on the
+launch stream) and the reference CPU path timed on this host's cores.
+
+Algorithmic bytes per entry (xbatched): 8*(n^d [X] + 2*n^d [Y RMW] + d*n^2
+[A]) + (d+2)*8 [pointers] = 99,136 B at d=6, n=4.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _splitmix_uniform(count, seed):
+    from helpers import uniform
+    return uniform(count, seed)
+
+
+def _splitmix_index(count, seed, mod):
+    from helpers import splitmix64
+    return (splitmix64(count, seed) % np.uint64(mod)).astype(np.int64)
+
+
+def make_xbatched_inputs(d, n, batch, nx, ny, seed):
+    """Host arrays of the headline workload (pools + maps)."""
+    E = n ** d
+    A = _splitmix_uniform(batch * d * n * n, seed * 7 + 1).reshape(batch * d, n * n)
+    X = _splitmix_uniform(nx * E, seed * 7 + 2).reshape(nx, E)
+    x_map = _splitmix_index(batch, seed * 7 + 3, nx)
+    y_map = _splitmix_index(batch, seed * 7 + 4, ny)
+    return A, X, x_map, y_map
+
+
+def bytes_per_entry(d, n, ptr_layout=True, elem=8):
+    b = elem * (n ** d + 2 * n ** d + d * n * n)
+    return b + ((d + 2) * 8 if ptr_layout else 0)
+
+
+def cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample):
+    """The reference CPU path (oracle/_ref, compiled from the reference's own
+    sources) or, where absent, the oracle's C restatement, on the first
+    `sample` entries of the same workload; best of 3."""
+    import oracle
+    L = oracle.ref_lib()
+    kind = "reference"
+    if L is None:
+        L = oracle.oracle_lib()
+        kind = "port"
+    E = n ** d
+    Y = np.zeros((ny, E))
+    subbatch = 1024
+    W = np.zeros((2 * subbatch, E))
+    a_map = np.arange(sample * d).reshape(sample, d)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.xbatched(L, d, n, A, a_map, n, X, x_map[:sample], Y, y_map[:sample], W,
+                        np.arange(2 * subbatch), sample, subbatch)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {
+        "value": sample / best,
+        "unit": "entries/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": "first %d entries of the same workload (same pools and maps), "
+                  "subbatchCount=1024, OpenMP threads=%d, best of 3 (%.2f s each)"
+                  % (sample, threads, best),
+    }
+
+
+def read_traffic(workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def bench_xbatched(args):
+    import torch
+
+    import kronmult_amd as K
+
+    d, n, batch, nx, ny = 6, 4, 65536, 8192, 2048
+    E = n ** d
+    dev = "cuda:0"
+    A, X, x_map, y_map = make_xbatched_inputs(d, n, batch, nx, ny, seed=2024)
+    Ad = torch.from_numpy(A).to(dev)
+    Xd = torch.from_numpy(X).to(dev)
+    Yd = torch.zeros((ny, E), dtype=torch.float64, device=dev)
+    nslot = batch  # pW: 2*subbatch slots (only touched by shapes larger than LDS)
+    Wd = torch.empty((1, E), dtype=torch.float64, device=dev)
+
+    def rows(t, idx):
+        base = t.data_ptr()
+        rb = t.stride(0) * t.element_size()
+        return torch.from_numpy(base + np.asarray(idx, dtype=np.int64) * rb).to(dev)
+
+    pA = rows(Ad, np.arange(batch * d))
+    pX = rows(Xd, x_map)
+    pY = rows(Yd, y_map)
+    pW = rows(Wd, np.zeros(nslot, dtype=np.int64))
+    stream = torch.cuda.current_stream()
+
+    def step():
+        K.kronmult_xbatched(d, n, pA, n, pX, pY, pW, batch, 0, dtype=torch.float64,
+                            stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev0.elapsed_time(ev1)
+    ms = ms_total / args.steps
+    value = batch / (ms * 1e-3)
+    alg_bytes = bytes_per_entry(d, n) * batch
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = read_traffic("xbatched_d6n4")
+    out = {
+        "metric": "batch entries/sec (kronmult6_xbatched d=6 n=4 FP64, batchCount 65536)",
+        "value": value,
+        "unit": "entries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (splitmix64 uniform [-1,1), seed 2024)",
+        "config": {"workload": "kronmult6_xbatched d=6 n=4 batchCount=65536 "
+                               "Xpool=8192 Ypool=2048 lda=4",
+                   "global_batch": batch},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "hbm_alg_gbs": achieved,
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample
+        cpu = cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample)
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def _time_steps(step, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def bench_batched(args):
+    """kronmult6_batched at the same d=6, n=4 FP64 and 65,536 entries
+    (contiguous layout, Y private per entry)."""
+    import torch
+
+    import kronmult_amd as K
+
+    d, n, batch = 6, 4, 65536
+    E = n ** d
+    dev = "cuda:0"
+    A = torch.from_numpy(_splitmix_uniform(batch * d * n * n, 11)).to(dev)
+    X = torch.from_numpy(_splitmix_uniform(batch * E, 12)).to(dev)
+    Y = torch.zeros(batch * E, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    ms = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, None, batch, stream=stream),
+                     args.steps, args.warmup, stream)
+    alg = bytes_per_entry(d, n, ptr_layout=False) * batch
+    print(json.dumps({"workload": "kronmult6_batched d=6 n=4 batchCount=65536", "dtype": "f64",
+                      "ms_per_step": ms, "entries_per_s": batch / (ms * 1e-3),
+                      "alg_gbs": alg / (ms * 1e-3) / 1e9,
+                      "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}), flush=True)
+
+
+def bench_copy(args):
+    """Device-to-device copy of 2 GiB: the achievable HBM read+write rate."""
+    import torch
+    n = 1 << 28  # doubles
+    a = torch.empty(n, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    ms = _time_steps(lambda: b.copy_(a), args.steps, args.warmup, stream)
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    print(json.dumps({"workload": "copy 2 GiB (read + write)", "ms": ms, "gbs": gbs,
+                      "frac": gbs / HBM_PEAK_GBS}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="xbatched_d6n4")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=16384)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "xbatched_d6n4":
+        bench_xbatched(args)
+    elif args.workload == "batched_d6n4":
+        bench_batched(args)
+    elif args.workload == "copy":
+        bench_copy(args)
+    else:
+        print("unknown workload %s" % args.workload, file=sys.stderr)
+        sys.exit(2)
+
+
+if __name__ == "__main__":
+    main()

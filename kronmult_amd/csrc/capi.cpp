@@ -2,7 +2,9 @@
 // (include/kronmult.hpp).  Host code only: validates arguments, applies the
 // reference's workspace rules, and hands a KronArgs to the kernel launchers.
 #include <cstdio>
+#include <map>
 #include <mutex>
+#include <utility>
 
 #include "kron_internal.hpp"
 #include "../../include/kronmult.hpp"
@@ -27,6 +29,56 @@ int max_lds_bytes()
         cached[dev] = v;
     }
     return cached[dev];
+}
+
+long long persistent_blocks(const void *kernel, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, long long> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(kernel, dev * 1048576 + (int)lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 0, per_cu = 0;
+    long long r = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) == hipSuccess &&
+        cus > 0 && per_cu > 0)
+        r = (long long)cus * per_cu;
+    cache[key] = r;
+    return r;
+}
+
+void *stream_workspace(hipStream_t stream, size_t bytes)
+{
+    struct Buf {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, Buf> bufs;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    Buf &b = bufs[std::make_pair(dev, stream)];
+    if (b.bytes < bytes) {
+        // the old buffer may still be in use by queued work on this stream
+        if (b.ptr) {
+            if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+            (void)hipFree(b.ptr);
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        if (hipMalloc(&b.ptr, want) != hipSuccess) {
+            b.ptr = nullptr;
+            return nullptr;
+        }
+        b.bytes = want;
+    }
+    return b.ptr;
 }
 
 // n^d with overflow check against the reference's int sizes.

@@ -30,16 +30,84 @@ constexpr int pow2ceil(int v)
     return p;
 }
 
-// Hardware FP atomic add (global_atomic_add_f64 / _f32 on gfx950).  Y must be
-// ordinary device memory (hipMalloc / torch), where these are exact adds.
-__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+// Pointers in the global address space: loads become global_load with an
+// SGPR base and a 32-bit VGPR offset, atomics global_atomic_add_{f64,f32}.
+// (Pointers read from the pointer arrays are otherwise generic -> flat_*.)
+template <typename T>
+struct GlobalAS {
+    typedef __attribute__((address_space(1))) T type;
+};
+template <typename T>
+using gptr = typename GlobalAS<T>::type *;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p)
+{
+    return (gptr<T>)p;
+}
+
+// Read-only inputs (factors, pointer arrays) through the constant address
+// space: a workgroup-uniform address then becomes a scalar load (s_load),
+// counted by lgkmcnt rather than vmcnt.  That matters: vmcnt retires in
+// order, so a vector load of a factor issued after a prefetch of the next X
+// would make the wait for the factor also wait for the whole prefetch.  The
+// kernels never write these arrays, which is what the address space asserts.
+template <typename T>
+struct ConstAS {
+    typedef __attribute__((address_space(4))) const T type;
+};
+template <typename T>
+using cptr = typename ConstAS<T>::type *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> to_const(const T *p)
+{
+    return (cptr<T>)p;
+}
+
+// A workgroup-uniform value moved to SGPRs (v_readfirstlane).  Only valid
+// when every active lane holds the same value.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    return reinterpret_cast<T *>(uniform_u64(reinterpret_cast<unsigned long long>(p)));
+}
+__device__ __forceinline__ double uniform_val(double v)
+{
+    return __builtin_bit_cast(double, uniform_u64(__builtin_bit_cast(unsigned long long, v)));
+}
+__device__ __forceinline__ float uniform_val(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, v)));
+}
+
+// Hardware FP atomic add (no-return global_atomic_add_f64 / _f32 on gfx950).
+// Y must be ordinary device memory (hipMalloc / torch), where these are exact
+// read-modify-write adds at device scope.
+__device__ __forceinline__ void atomic_add(gptr<double> p, double v)
+{
+    __builtin_amdgcn_global_atomic_fadd_f64(p, v);
+}
+__device__ __forceinline__ void atomic_add(gptr<float> p, float v)
+{
+    __builtin_amdgcn_global_atomic_fadd_f32(p, v);
+}
+__device__ __forceinline__ void atomic_add(double *p, double v) { atomic_add(to_global(p), v); }
+__device__ __forceinline__ void atomic_add(float *p, float v) { atomic_add(to_global(p), v); }
 
 // Per-entry addressing of the two argument layouts.
 template <typename T, Layout L>
 struct Entries;
 
 // kronmult6_batched.hpp:51-71: A (n,n,d,batch), X/Y (size, batch).
+// The *_g accessors return global-address-space pointers; UNI asserts that b
+// is the same on every lane of the workgroup (one entry per workgroup).
 template <typename T>
 struct Entries<T, Layout::Batched> {
     const T *A;
@@ -54,6 +122,18 @@ struct Entries<T, Layout::Batched> {
     }
     __device__ __forceinline__ const T *x(int b) const { return X + (long long)b * sx; }
     __device__ __forceinline__ T *y(int b) const { return Y + (long long)b * sy; }
+    template <bool UNI>
+    __device__ __forceinline__ auto factor_g(int b, int k) const
+    {
+        if constexpr (UNI)
+            return to_const(factor(b, k));
+        else
+            return to_global(factor(b, k));
+    }
+    template <bool UNI>
+    __device__ __forceinline__ gptr<const T> x_g(int b) const { return to_global(x(b)); }
+    template <bool UNI>
+    __device__ __forceinline__ gptr<T> y_g(int b) const { return to_global(y(b)); }
 };
 
 // kronmult_xbatched.hpp:79-83,123-135: Aarray[k + b*d], pX[b], pY[b].
@@ -69,11 +149,35 @@ struct Entries<T, Layout::Ptr> {
     }
     __device__ __forceinline__ const T *x(int b) const { return pX[b]; }
     __device__ __forceinline__ T *y(int b) const { return pY[b]; }
+    template <bool UNI>
+    __device__ __forceinline__ auto factor_g(int b, int k) const
+    {
+        if constexpr (UNI)
+            return to_const(to_const(pA)[(long long)b * d + k]);
+        else
+            return to_global(factor(b, k));
+    }
+    template <bool UNI>
+    __device__ __forceinline__ gptr<const T> x_g(int b) const
+    {
+        if constexpr (UNI)
+            return to_global(to_const(pX)[b]);
+        else
+            return to_global(x(b));
+    }
+    template <bool UNI>
+    __device__ __forceinline__ gptr<T> y_g(int b) const
+    {
+        if constexpr (UNI)
+            return to_global(to_const(pY)[b]);
+        else
+            return to_global(y(b));
+    }
 };
 
 // Y(idx) += v: device atomic when entries may share Y, else load-add-store.
-template <bool SHARED, typename T, typename Acc>
-__device__ __forceinline__ void y_accumulate(T *y, long long idx, Acc v)
+template <bool SHARED, typename T, typename Acc, typename P>
+__device__ __forceinline__ void y_accumulate(P y, long long idx, Acc v)
 {
     if constexpr (SHARED) {
         atomic_add(y + idx, T(v));

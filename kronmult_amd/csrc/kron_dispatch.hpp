@@ -3,6 +3,7 @@
 #pragma once
 
 #include "kron_device.hpp"
+#include "kron_fast.hpp"
 
 namespace kron {
 
@@ -11,10 +12,11 @@ kron_status fast_batched_f64(int D, int N, const Entries<double, Layout::Batched
                              bool trans, int batch, hipStream_t s);
 kron_status fast_batched_f32(int D, int N, const Entries<float, Layout::Batched> &e,
                              bool trans, int batch, hipStream_t s);
-// Pointer layout (Y may be shared -> atomics), leading dimension lda.
+// Pointer layout (Y may be shared -> atomics), leading dimension lda.  With
+// ord != nullptr the entries are visited in Y-target order and merged.
 kron_status fast_ptr_f64(int D, int N, const Entries<double, Layout::Ptr> &e, int lda,
-                         int batch, hipStream_t s);
+                         int batch, hipStream_t s, const GroupOrder *ord);
 kron_status fast_ptr_f32(int D, int N, const Entries<float, Layout::Ptr> &e, int lda,
-                         int batch, hipStream_t s);
+                         int batch, hipStream_t s, const GroupOrder *ord);
 
 }  // namespace kron

@@ -34,14 +34,18 @@ struct FastGeom {
 };
 
 // a[i][j] = A(i,j) (or A(j,i) when TRANS), A column-major with leading dim lda.
-template <typename T, typename Acc, int N, bool TRANS>
-__device__ __forceinline__ void load_factor(Acc (&a)[N][N], const T *__restrict__ Ap, int lda)
+// With one entry per workgroup Ap is a constant-address-space pointer: the
+// factor is read with scalar loads into SGPRs (v_fma_f64 takes one scalar
+// operand), saving N*N*2 VGPRs.
+template <typename T, typename Acc, int N, bool TRANS, bool UNI, typename PtrT>
+__device__ __forceinline__ void load_factor(Acc (&a)[N][N], PtrT Ap, int lda)
 {
 #pragma unroll
     for (int j = 0; j < N; ++j)
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-            a[i][j] = TRANS ? Acc(Ap[j + i * lda]) : Acc(Ap[i + j * lda]);
+        for (int i = 0; i < N; ++i) {
+            a[i][j] = Acc(TRANS ? Ap[j + i * lda] : Ap[i + j * lda]);
+        }
 }
 
 // One pass over the fibers of mode K owned by thread g of its entry.
@@ -70,57 +74,244 @@ __device__ __forceinline__ void fiber_pass(int g, const Acc (&a)[N][N], LoadF &&
     }
 }
 
-template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS>
+// LDS address of element i.  For N = 4 (and at least 64 elements) the two
+// low 2-bit digits are XORed with the third: the 16 lanes that a 64-bit LDS
+// access serves together then hit 16 distinct 8-byte bank slots for every
+// mode's fiber pattern (strides 1, 4, 16, 64, ...).  Other N: identity.
+template <int D, int N>
+__device__ __forceinline__ int lds_index(int i)
+{
+    if constexpr (N == 4 && D >= 3) {
+        const int h = (i >> 4) & 3;
+        return i ^ (h | (h << 2));
+    } else {
+        return i;
+    }
+}
+
+// Registers of one thread's mode-2 fibers (the first pass reads X here).
+template <typename T, int D, int N, int G>
+struct XFibers {
+    static constexpr int F = cpow(N, D - 1);
+    static constexpr int S = cpow(N, D - 2);
+    static constexpr int IT = (F + G - 1) / G;
+    T v[IT][N];
+    template <typename PtrT>
+    __device__ __forceinline__ void load(PtrT x, int g)
+    {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = g + it * G;
+            if ((F % G) == 0 || f < F) {
+                const int base = (f % S) + (f / S) * (S * N);
+#pragma unroll
+                for (int j = 0; j < N; ++j) v[it][j] = x[base + j * S];
+            }
+        }
+    }
+};
+
+// Prefetching the next entry's X costs IT*N registers; worth it while that
+// stays within 32 VGPRs.
+template <typename T, int D, int N>
+constexpr bool fast_pipelined()
+{
+    using X = XFibers<T, D, N, FastGeom<D, N>::G>;
+    return D >= 2 && (size_t)X::IT * N * sizeof(T) <= 128;
+}
+
+// Entry order for grouped calls: perm[r] is the r-th entry in Y-target order,
+// keys[r] its target (pY[perm[r]]); workgroup w takes positions
+// [w*chunk, (w+1)*chunk).
+struct GroupOrder {
+    const int *perm;
+    const unsigned long long *keys;
+    int chunk;
+};
+
+// Grouping needs one entry per workgroup and the mode-1 outputs of a thread
+// (the running Y sum) within 32 VGPRs.
+template <typename T, int D, int N>
+constexpr bool fast_groupable()
+{
+    using Geo = FastGeom<D, N>;
+    return D >= 2 && Geo::P == 1 &&
+           (size_t)((Geo::F + Geo::G - 1) / Geo::G) * N * sizeof(T) <= 128;
+}
+
+// Persistent workgroups walk the batch with stride gridDim.x * P.  With
+// PIPE the loads of the next entry's X are issued before the current entry's
+// LDS passes, so HBM latency overlaps the on-chip work.
+// GROUPED: entries are visited in Y-target order (GroupOrder); the mode-1
+// results of consecutive entries with the same target are summed in registers
+// and added to Y once per run ("batched atomicAdd merges").
+template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS,
+          bool GROUPED>
 __global__ void __launch_bounds__(256)
-kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch)
+kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
 {
     using Geo = FastGeom<D, N>;
     constexpr int E = Geo::E;
     constexpr int G = Geo::G;
     constexpr int P = Geo::P;
+    constexpr bool PIPE = fast_pipelined<T, D, N>();
+    constexpr bool UNI = (P == 1);  // one entry per workgroup: pointers are uniform
     extern __shared__ __align__(16) unsigned char kron_smem[];
     T *smem = reinterpret_cast<T *>(kron_smem);
 
     const int tid = threadIdx.x;
     const int slot = (P == 1) ? 0 : tid / G;
-    const int g = (P == 1) ? tid : tid % G;
-    const int b = blockIdx.x * P + slot;
-    const bool active = b < batch;
+    const int g0 = (P == 1) ? tid : tid % G;
+    const int stride = gridDim.x * P;
     Acc a[N][N];
+    static_assert(!GROUPED || fast_groupable<T, D, N>(), "grouping needs P == 1, D >= 2");
+
+    // positions r in [rbeg, rend) with step rstep; entry(r) is the batch index
+    int rbeg, rend, rstep;
+    if constexpr (GROUPED) {
+        rbeg = blockIdx.x * ord.chunk;
+        rend = min(batch, rbeg + ord.chunk);
+        rstep = 1;
+    } else {
+        rbeg = blockIdx.x * P;
+        rend = batch;
+        rstep = stride;
+    }
+    auto entry = [&](int r) -> int {
+        if constexpr (GROUPED)
+            return to_const(ord.perm)[r];
+        else
+            return r + slot;
+    };
 
     if constexpr (D == 1) {
-        if (active) {
-            load_factor<T, Acc, N, TRANS>(a, ent.factor(b, 0), lda);
-            const T *__restrict__ x = ent.x(b);
-            T *y = ent.y(b);
+        const int g = g0;
+        for (int b = blockIdx.x * P + slot; b < batch; b += stride) {
+            load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 0), lda);
+            const auto x = ent.template x_g<UNI>(b);
+            const auto y = ent.template y_g<UNI>(b);
             fiber_pass<Acc, D, N, 1, G>(
                 g, a, [&](int i) { return x[i]; },
                 [&](int i, Acc v) { y_accumulate<SHARED, T, Acc>(y, i, v); });
         }
     } else {
         T *buf = smem + slot * E;
-        if (active) {
-            load_factor<T, Acc, N, TRANS>(a, ent.factor(b, 1), lda);
-            const T *__restrict__ x = ent.x(b);
-            fiber_pass<Acc, D, N, 2, G>(
-                g, a, [&](int i) { return x[i]; }, [&](int i, Acc v) { buf[i] = T(v); });
+        auto L = [&](int i) -> T & { return buf[lds_index<D, N>(i)]; };
+        XFibers<T, D, N, (PIPE ? G : 1 << 30)> nxt;  // empty unless PIPE
+        if constexpr (PIPE) {
+            if (rbeg < rend) {
+                const int b = entry(rbeg);
+                if (b < batch) nxt.load(ent.template x_g<UNI>(b), g0);
+            }
         }
-        __syncthreads();
-        for_modes_desc<D, 3>([&](auto kc) {
-            constexpr int K = decltype(kc)::value;
-            if (active) {
-                load_factor<T, Acc, N, TRANS>(a, ent.factor(b, K - 1), lda);
-                fiber_pass<Acc, D, N, K, G>(
-                    g, a, [&](int i) { return buf[i]; }, [&](int i, Acc v) { buf[i] = T(v); });
+        constexpr int F1 = cpow(N, D - 1);   // mode-1 fibers (= stride of mode 1)
+        constexpr int IT1 = (F1 + G - 1) / G;
+        Acc acc[GROUPED ? IT1 : 1][N];       // running Y sum of the current run
+        if constexpr (GROUPED) {
+#pragma unroll
+            for (int q = 0; q < IT1; ++q)
+#pragma unroll
+                for (int i = 0; i < N; ++i) acc[q][i] = 0;
+        }
+        for (int r = rbeg; r < rend; r += rstep) {
+            const int b = entry(r);
+            const bool active = GROUPED || b < batch;
+            // Keep the lane index opaque per iteration: otherwise every LDS and
+            // global address of every mode pass is hoisted out of the loop and
+            // held in ~100 VGPRs, halving occupancy.
+            int g = g0;
+            asm volatile("" : "+v"(g));
+            if constexpr (PIPE) {
+                // mode 2 from the registers loaded one iteration ahead; the
+                // same registers then receive the next entry's X, whose loads
+                // stay in flight during the remaining passes.
+                if (active) {
+                    load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 1), lda);
+                    constexpr int S2 = cpow(N, D - 2);
+                    constexpr int F = cpow(N, D - 1);
+#pragma unroll
+                    for (int q = 0; q < XFibers<T, D, N, G>::IT; ++q) {
+                        const int f = g + q * G;
+                        if ((F % G) == 0 || f < F) {
+                            const int base = (f % S2) + (f / S2) * (S2 * N);
+#pragma unroll
+                            for (int i = 0; i < N; ++i) {
+                                Acc s = a[i][0] * Acc(nxt.v[q][0]);
+#pragma unroll
+                                for (int j = 1; j < N; ++j)
+                                    s = __builtin_fma(a[i][j], Acc(nxt.v[q][j]), s);
+                                L(base + i * S2) = T(s);
+                            }
+                        }
+                    }
+                }
+                if (r + rstep < rend) {
+                    const int bn = entry(r + rstep);
+                    if (GROUPED || bn < batch) nxt.load(ent.template x_g<UNI>(bn), g);
+                }
+            } else {
+                if (active) {
+                    load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 1), lda);
+                    const auto x = ent.template x_g<UNI>(b);
+                    fiber_pass<Acc, D, N, 2, G>(
+                        g, a, [&](int i) { return x[i]; }, [&](int i, Acc v) { L(i) = T(v); });
+                }
             }
             __syncthreads();
-        });
-        if (active) {
-            load_factor<T, Acc, N, TRANS>(a, ent.factor(b, 0), lda);
-            T *y = ent.y(b);
-            fiber_pass<Acc, D, N, 1, G>(
-                g, a, [&](int i) { return buf[i]; },
-                [&](int i, Acc v) { y_accumulate<SHARED, T, Acc>(y, i, v); });
+            for_modes_desc<D, 3>([&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                if (active) {
+                    load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, K - 1),
+                                                       lda);
+                    fiber_pass<Acc, D, N, K, G>(
+                        g, a, [&](int i) { return L(i); }, [&](int i, Acc v) { L(i) = T(v); });
+                }
+                __syncthreads();
+            });
+            if constexpr (GROUPED) {
+                load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 0), lda);
+#pragma unroll
+                for (int q = 0; q < IT1; ++q) {
+                    const int f = g + q * G;
+                    if ((F1 % G) == 0 || f < F1) {
+                        Acc in[N];
+#pragma unroll
+                        for (int j = 0; j < N; ++j) in[j] = Acc(L(f + j * F1));
+#pragma unroll
+                        for (int i = 0; i < N; ++i) {
+                            Acc s = a[i][0] * in[0];
+#pragma unroll
+                            for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+                            acc[q][i] += s;
+                        }
+                    }
+                }
+                const auto keys = to_const(ord.keys);
+                const unsigned long long key = keys[r];
+                if (r + 1 >= rend || keys[r + 1] != key) {
+                    const auto y = to_global(reinterpret_cast<T *>(key));
+#pragma unroll
+                    for (int q = 0; q < IT1; ++q) {
+                        const int f = g + q * G;
+                        if ((F1 % G) == 0 || f < F1) {
+#pragma unroll
+                            for (int i = 0; i < N; ++i) {
+                                atomic_add(y + (f + i * F1), T(acc[q][i]));
+                                acc[q][i] = 0;
+                            }
+                        }
+                    }
+                }
+            } else {
+                if (active) {
+                    load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 0), lda);
+                    const auto y = ent.template y_g<UNI>(b);
+                    fiber_pass<Acc, D, N, 1, G>(
+                        g, a, [&](int i) { return L(i); },
+                        [&](int i, Acc v) { y_accumulate<SHARED, T, Acc>(y, i, v); });
+                }
+            }
+            __syncthreads();  // LDS is rewritten by the next entry
         }
     }
 }
@@ -135,23 +326,59 @@ constexpr int fast_max_n(int D)
 }
 
 template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS>
-kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipStream_t stream)
+kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipStream_t stream,
+                            const GroupOrder *ord)
 {
     using Geo = FastGeom<D, N>;
     const size_t lds = (D == 1) ? 0 : Geo::lds_bytes(sizeof(T));
     if ((long long)lds > (long long)max_lds_bytes()) return KRON_ENOTSUP;
-    auto kern = kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS>;
+    constexpr bool CAN_GROUP = SHARED && fast_groupable<T, D, N>();
+    const bool grouped = CAN_GROUP && ord != nullptr;
+    const void *kern;
+    if constexpr (CAN_GROUP) {
+        kern = grouped ? reinterpret_cast<const void *>(
+                             kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, true>)
+                       : reinterpret_cast<const void *>(
+                             kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>);
+    } else {
+        kern = reinterpret_cast<const void *>(
+            kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>);
+    }
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
         if (e != hipSuccess) {
             set_last_hip_error((int)e);
             return KRON_EHIP;
         }
     }
-    const long long nblocks = ((long long)batch + Geo::P - 1) / Geo::P;
+    const long long resident = persistent_blocks(kern, lds);
+    GroupOrder o{nullptr, nullptr, 0};
+    long long nblocks = ((long long)batch + Geo::P - 1) / Geo::P;
+    if (grouped) {
+        // equal contiguous chunks of the target order, one per resident block
+        const long long blocks = resident > 0 ? resident : 1024;
+        const long long chunk = ((long long)batch + blocks - 1) / blocks;
+        o = *ord;
+        o.chunk = (int)chunk;
+        nblocks = ((long long)batch + chunk - 1) / chunk;
+    } else if (resident > 0 && nblocks > resident) {
+        nblocks = resident;
+    }
     if (nblocks > 0x7fffffffLL) return KRON_EINVAL;
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch);
+    if constexpr (CAN_GROUP) {
+        if (grouped)
+            hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, true>),
+                               dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
+                               o);
+        else
+            hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>),
+                               dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
+                               o);
+    } else {
+        hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>),
+                           dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch, o);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_last_hip_error((int)e);
@@ -160,17 +387,32 @@ kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipS
     return KRON_OK;
 }
 
+// Whether (D, N) has a grouped (Y-merging) fast kernel for element type T.
+template <typename T>
+bool fast_groupable_rt(int D, int N)
+{
+#define KRON_G_CASE(DD, NN) \
+    if (D == DD && N == NN) return fast_groupable<T, DD, NN>();
+#define KRON_G_D(DD) \
+    KRON_G_CASE(DD, 1) KRON_G_CASE(DD, 2) KRON_G_CASE(DD, 3) KRON_G_CASE(DD, 4) \
+    KRON_G_CASE(DD, 5) KRON_G_CASE(DD, 6) KRON_G_CASE(DD, 7) KRON_G_CASE(DD, 8)
+    KRON_G_D(2) KRON_G_D(3) KRON_G_D(4) KRON_G_D(5) KRON_G_D(6)
+#undef KRON_G_D
+#undef KRON_G_CASE
+    return false;
+}
+
 // Runtime (D, N) -> compiled kernel.  KRON_ENOTSUP when N > fast_max_n.
 template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS>
 kron_status launch_fast(int D, int N, const Entries<T, LAY> &ent, int lda, int batch,
-                        hipStream_t stream)
+                        hipStream_t stream, const GroupOrder *ord = nullptr)
 {
     if (N < 1 || N > fast_max_n<T>(D)) return KRON_ENOTSUP;
 #define KRON_FAST_CASE(DD, NN)                                                           \
     if (D == DD && N == NN) {                                                            \
         if constexpr (NN <= fast_max_n<T>(DD))                                           \
             return launch_fast_one<T, Acc, DD, NN, LAY, SHARED, TRANS>(ent, lda, batch,  \
-                                                                       stream);          \
+                                                                       stream, ord);     \
         else                                                                             \
             return KRON_ENOTSUP;                                                         \
     }

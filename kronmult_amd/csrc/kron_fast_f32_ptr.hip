@@ -5,9 +5,9 @@
 namespace kron {
 
 kron_status fast_ptr_f32(int D, int N, const Entries<float, Layout::Ptr> &e, int lda,
-                         int batch, hipStream_t s)
+                         int batch, hipStream_t s, const GroupOrder *ord)
 {
-    return launch_fast<float, float, Layout::Ptr, true, false>(D, N, e, lda, batch, s);
+    return launch_fast<float, float, Layout::Ptr, true, false>(D, N, e, lda, batch, s, ord);
 }
 
 }  // namespace kron

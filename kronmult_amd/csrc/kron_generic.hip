@@ -270,6 +270,29 @@ size_t kron_scratch_elems(const KronArgs<T> &a)
     return sh.h == 0 ? 0 : (size_t)(sh.nhi_out * sh.nlo_in);
 }
 
+// Entries of a pointer-layout call that share Y targets are merged when the
+// batch is large enough for the sort to pay off and the shape has a grouped
+// kernel.  Failure to get the workspace falls back to per-entry atomics.
+constexpr int kGroupMinBatch = 2048;
+
+template <typename T>
+static bool group_order(const KronArgs<T> &a, int N, GroupOrder &ord, hipStream_t s)
+{
+    if (!a.y_shared || a.batch < kGroupMinBatch || a.trans || !fast_groupable_rt<T>(a.d, N))
+        return false;
+    const size_t bytes = group_workspace_bytes(a.batch);
+    void *ws = stream_workspace(s, bytes);
+    if (!ws) return false;
+    const unsigned long long *keys = nullptr;
+    const int *perm = nullptr;
+    if (group_by_target(reinterpret_cast<const void *const *>(a.pY), a.batch, ws, bytes, &keys,
+                        &perm, s) != KRON_OK)
+        return false;
+    ord.perm = perm;
+    ord.keys = keys;
+    return true;
+}
+
 static kron_status fast_dispatch(const KronArgs<double> &a, int N, hipStream_t s)
 {
     if (a.layout == Layout::Batched) {
@@ -280,7 +303,9 @@ static kron_status fast_dispatch(const KronArgs<double> &a, int N, hipStream_t s
         return fast_batched_f64(a.d, N, e, a.trans, a.batch, s);
     }
     Entries<double, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
-    return fast_ptr_f64(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s);
+    GroupOrder ord{nullptr, nullptr, 0};
+    const bool grouped = group_order(a, N, ord, s);
+    return fast_ptr_f64(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s, grouped ? &ord : nullptr);
 }
 
 static kron_status fast_dispatch(const KronArgs<float> &a, int N, hipStream_t s)
@@ -293,7 +318,9 @@ static kron_status fast_dispatch(const KronArgs<float> &a, int N, hipStream_t s)
         return fast_batched_f32(a.d, N, e, a.trans, a.batch, s);
     }
     Entries<float, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
-    return fast_ptr_f32(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s);
+    GroupOrder ord{nullptr, nullptr, 0};
+    const bool grouped = group_order(a, N, ord, s);
+    return fast_ptr_f32(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s, grouped ? &ord : nullptr);
 }
 
 template <typename T>

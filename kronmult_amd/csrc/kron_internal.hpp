@@ -72,8 +72,21 @@ kron_status launch_kgemm(bool transB, int mm, int nn, int kk, T alpha,
                          const int *ldB, T beta, T *const *C, const int *ldC,
                          int batch, hipStream_t stream);
 
+// Y-target ordering of pointer-layout entries (kron_group.hip).
+size_t group_workspace_bytes(int batch);
+kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t ws_bytes,
+                            const unsigned long long **keys_sorted, const int **perm,
+                            hipStream_t stream);
+// Device scratch owned by the library, one buffer per (device, stream),
+// grown on demand and kept for the life of the process.  Work on one stream
+// is ordered, so calls on the same stream may reuse it.  nullptr on failure.
+void *stream_workspace(hipStream_t stream, size_t bytes);
+
 // Per-thread record of the last HIP error (kron_last_hip_error()).
 void set_last_hip_error(int code);
 int max_lds_bytes();
+// Workgroups of `kernel` (256 threads, `lds` dynamic bytes) that fit on the
+// current device at once: CUs x resident blocks per CU (0 if unknown).
+long long persistent_blocks(const void *kernel, size_t lds);
 
 }  // namespace kron

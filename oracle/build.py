@@ -20,6 +20,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.environ.get("KRON_REFERENCE_DIR", "/root/reference")
+LLVM_LIB = "/opt/rocm/llvm/lib"  # clang's OpenMP runtime (libomp)
 
 REF_SOURCES = (
     [f"kronmult{d}_batched.cpp" for d in range(1, 7)]
@@ -37,7 +38,7 @@ def _run(cmd):
 
 def build_oracle():
     out = os.path.join(HERE, "liboracle.so")
-    _run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+    _run(["gcc", "-O3", "-std=c11", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
           os.path.join(HERE, "kron_oracle.c"), "-o", out])
     return out
 
@@ -51,9 +52,10 @@ def build_ref():
     out = os.path.join(outdir, "libkron_ref.so")
     srcs = [os.path.join(REF_DIR, s) for s in REF_SOURCES]
     cmd = ["hipcc", "-x", "hip", "--offload-host-only", "-fdelayed-template-parsing",
-           "-O2", "-ffp-contract=off", "-fopenmp=libgomp", "-fPIC", "-shared",
+           "-O3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
            "-w", "-I" + REF_DIR] + srcs + [os.path.join(HERE, "ref_shim.cpp"),
-                                           "-Wl,--allow-multiple-definition", "-o", out]
+                                           "-Wl,--allow-multiple-definition",
+                                           "-Wl,-rpath," + LLVM_LIB, "-o", out]
     _run(cmd)
     return out
 

@@ -337,3 +337,47 @@ def test_kgemm_shared_c_atomic():
                        ptrs(Bd, np.arange(batch)), ints([kk] * batch), 1.0,
                        ptrs(Cd, np.zeros(batch)), ints([mm] * batch), batch)
     assert np.array_equal(host(Cd)[0], want.reshape(-1))
+
+
+# --- grouped (Y-merging) pointer-layout path: batch >= 2048 -------------------
+
+@pytest.mark.parametrize("d,n", [(6, 4), (5, 4), (5, 5), (4, 5)])
+@pytest.mark.parametrize("ny", [1, 3, 257, 0])
+def test_xbatched_grouped_integer_bit_exact(d, n, ny):
+    """Entries sorted by Y target and merged in registers; ny = 0 means every
+    entry has its own Y (no sharing).  Integer inputs: exact."""
+    batch = 2500 if n ** d <= 4096 else 2100
+    ny_eff = batch if ny == 0 else ny
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 97, ny_eff, n, 21 + ny,
+                                                  kind="int")
+    if ny == 0:
+        y_map = np.random.default_rng(5).permutation(batch)
+    Y0 = H.integers(ny_eff * n ** d, 3, -5, 5).reshape(ny_eff, -1)
+    got = _gpu_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_xbatched_grouped_uniform(dtype):
+    d, n, batch, ny = 6, 4, 4096, 40
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 300, ny, n, 8)
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, n, A_pool.astype(dtype), X_pool.astype(dtype), x_map,
+                        Y0.astype(dtype), y_map)
+    want = _oracle_xbatched(d, n, n, A_pool.astype(dtype).astype(np.float64),
+                            X_pool.astype(dtype).astype(np.float64), x_map, Y0, y_map)
+    assert H.rel_err(got, want) < (F64_TOL if dtype == np.float64 else F32_TOL)
+
+
+def test_xbatched_repeat_calls_accumulate():
+    """Two calls on one stream reuse the library workspace; Y accumulates."""
+    d, n, batch, ny = 6, 4, 3000, 11
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 64, ny, n, 12, kind="int")
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(np.zeros((ny, n ** d)))
+    pA = ptrs(Ad, np.arange(batch * d))
+    pX, pY = ptrs(Xd, x_map), ptrs(Yd, y_map)
+    for _ in range(2):
+        K.kronmult6_xbatched(n, pA, n, pX, pY, pX, batch)
+    want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, np.zeros((ny, n ** d)), y_map)
+    assert np.array_equal(host(Yd), 2 * want)
