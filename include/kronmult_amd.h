@@ -12,8 +12,11 @@
  *     element-wise device atomic add, as in the reference
  *     (kgemm_nn.hpp:392-394);
  *   - every data pointer and every pointer/leading-dimension array is device
- *     memory owned by the caller; the library allocates nothing on these
- *     paths and never synchronises the stream;
+ *     memory owned by the caller; calls never synchronise the stream;
+ *   - pointer-layout calls with batchCount >= 2048 order their entries by Y
+ *     target on the device and merge runs before the atomic adds; the sort
+ *     scratch (about 12 bytes per entry) is a library-owned device buffer per
+ *     (device, stream), allocated on first use and kept for the process;
  *   - X is never written and W is used only as scratch when a shape does not
  *     fit on chip (the reference clobbers X in its batched layouts; callers
  *     must still treat X/W contents as unspecified after a call there).
@@ -101,6 +104,31 @@ kron_status kron_vbatched_f32(int d, const int *m, const int *n,
                               const float *const *pX, float *const *pY,
                               float *W, size_t Wcapacity_bytes, int batchCount,
                               kron_stream_t stream);
+
+/*
+ * xbatched plan (no counterpart in the reference; the reference re-derives
+ * nothing between calls).  For repeated applies with an unchanged pointer
+ * map, plan_create validates the arguments of kron_xbatched_* and computes,
+ * on `stream`, the entries' order by Y target; plan_execute then performs
+ * exactly the kron_xbatched_* update (Y += ..., atomic merges) using that
+ * order.  The contents of A, X and Y may change between executes; the
+ * pointer arrays Aarray, pX, pY must not.  plan_create allocates device
+ * memory (about 12 bytes per entry plus sort scratch); plan_destroy
+ * synchronises the device and frees it.
+ */
+typedef struct kron_xplan_s *kron_xplan;
+kron_status kron_xbatched_plan_create_f64(kron_xplan *plan, int d, int n,
+                                          const double *const *Aarray, int lda,
+                                          const double *const *pX, double *const *pY,
+                                          double *const *pW, int batchCount,
+                                          int subbatchCount, kron_stream_t stream);
+kron_status kron_xbatched_plan_create_f32(kron_xplan *plan, int d, int n,
+                                          const float *const *Aarray, int lda,
+                                          const float *const *pX, float *const *pY,
+                                          float *const *pW, int batchCount,
+                                          int subbatchCount, kron_stream_t stream);
+kron_status kron_xbatched_plan_execute(kron_xplan plan, kron_stream_t stream);
+kron_status kron_xbatched_plan_destroy(kron_xplan plan);
 
 /*
  * Batched small GEMM with per-entry pointers and leading dimensions (device

@@ -20,7 +20,7 @@ import functools
 import os
 
 __all__ = [
-    "KronError", "lib", "ptr_array",
+    "KronError", "lib", "ptr_array", "XbatchedPlan",
     "kronmult_batched", "kronmult_xbatched", "kronmult_vbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
 ] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
@@ -67,6 +67,15 @@ def lib():
             f.argtypes = [_c_int, _c_int, _c_int, real, _c_vp, _c_vp, _c_vp, _c_vp, real,
                           _c_vp, _c_vp, _c_int, _c_vp]
             f.restype = _c_int
+    for sfx in ("f64", "f32"):
+        f = getattr(L, "kron_xbatched_plan_create_" + sfx)
+        f.argtypes = [ctypes.POINTER(_c_vp), _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp,
+                      _c_int, _c_int, _c_vp]
+        f.restype = _c_int
+    L.kron_xbatched_plan_execute.argtypes = [_c_vp, _c_vp]
+    L.kron_xbatched_plan_execute.restype = _c_int
+    L.kron_xbatched_plan_destroy.argtypes = [_c_vp]
+    L.kron_xbatched_plan_destroy.restype = _c_int
     L.kron_status_string.argtypes = [_c_int]
     L.kron_status_string.restype = ctypes.c_char_p
     L.kron_last_hip_error.restype = _c_int
@@ -157,6 +166,37 @@ def kronmult_vbatched(m, n, Aarray, Xarray, Yarray, W, Wcapacity, batchCount,
     _check("kronmult%d_vbatched" % d,
            f(d, mm, nn, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(W), Wcapacity,
              batchCount, _stream(stream)))
+
+
+class XbatchedPlan:
+    """kron_xbatched_plan_*: the Y-target order of an xbatched call computed
+    once, reused by every execute() while the pointer arrays stay unchanged."""
+
+    def __init__(self, d, n, Aarray, lda, Xarray, Yarray, Warray, batchCount,
+                 subbatchCount=0, dtype=None, stream=None):
+        import torch
+        sfx = _sfx(dtype if dtype is not None else torch.float64)
+        self._h = _c_vp()
+        self._keep = (Aarray, Xarray, Yarray, Warray)
+        f = getattr(lib(), "kron_xbatched_plan_create_" + sfx)
+        _check("kron_xbatched_plan_create",
+               f(ctypes.byref(self._h), d, n, _addr(Aarray), lda, _addr(Xarray),
+                 _addr(Yarray), _addr(Warray), batchCount, subbatchCount, _stream(stream)))
+
+    def execute(self, stream=None):
+        _check("kron_xbatched_plan_execute",
+               lib().kron_xbatched_plan_execute(self._h, _stream(stream)))
+
+    def close(self):
+        if self._h:
+            lib().kron_xbatched_plan_destroy(self._h)
+            self._h = _c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _kgemm(tr, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray, beta, Carray,

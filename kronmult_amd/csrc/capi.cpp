@@ -119,7 +119,7 @@ static kron_status batched_impl(int d, int n, const T *A, const T *X, T *Y, T *W
 }
 
 template <typename T>
-static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
+static kron_status xbatched_args(KronArgs<T> &a, int d, int n, const T *const *Aarray, int lda,
                                  const T *const *pX, T *const *pY, T *const *pW, int batch,
                                  int subbatch, kron_stream_t stream)
 {
@@ -127,11 +127,9 @@ static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
     if (d < 1 || d > kMaxDim || n < 1 || lda < n || batch < 0 || subbatch < 0 ||
         !ipow_ok(n, d, E))
         return KRON_EINVAL;
-    if (batch == 0) return KRON_OK;
-    if (!Aarray || !pX || !pY) return KRON_EINVAL;
+    if (batch > 0 && (!Aarray || !pX || !pY)) return KRON_EINVAL;
     // kronmult_xbatched.hpp:73-75
     if (subbatch == 0) subbatch = (batch / 2 > 1) ? batch / 2 : 1;
-    KronArgs<T> a;
     a.d = d;
     for (int k = 0; k < d; ++k) a.m[k] = a.n[k] = n;
     a.lda = lda;
@@ -147,7 +145,18 @@ static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
     a.y_shared = true;
     a.batch = batch;
     a.stream = static_cast<hipStream_t>(stream);
-    if (kron_scratch_elems(a) > 0 && !pW) return KRON_EWORKSPACE;
+    if (batch > 0 && kron_scratch_elems(a) > 0 && !pW) return KRON_EWORKSPACE;
+    return KRON_OK;
+}
+
+template <typename T>
+static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
+                                 const T *const *pX, T *const *pY, T *const *pW, int batch,
+                                 int subbatch, kron_stream_t stream)
+{
+    KronArgs<T> a;
+    kron_status st = xbatched_args(a, d, n, Aarray, lda, pX, pY, pW, batch, subbatch, stream);
+    if (st != KRON_OK || batch == 0) return st;
     return launch_kron(a);
 }
 
@@ -224,6 +233,72 @@ static kron_status kgemm_impl(bool transB, int mm, int nn, int kk, T alpha,
 
 using namespace kron;
 
+// An xbatched plan: the call's arguments plus its Y-target order, computed
+// once on the device and reused by every execute.
+struct kron_xplan_s {
+    int elem = 0;  // 8 or 4
+    KronArgs<double> a64;
+    KronArgs<float> a32;
+    void *ws = nullptr;
+    GroupOrder ord{nullptr, nullptr, 0};
+    bool grouped = false;
+    int device = 0;
+};
+
+template <typename T>
+static KronArgs<T> &plan_args(kron_xplan_s *p);
+template <>
+KronArgs<double> &plan_args<double>(kron_xplan_s *p) { return p->a64; }
+template <>
+KronArgs<float> &plan_args<float>(kron_xplan_s *p) { return p->a32; }
+
+template <typename T>
+static kron_status plan_create(kron_xplan *out, int d, int n, const T *const *Aarray, int lda,
+                               const T *const *pX, T *const *pY, T *const *pW, int batch,
+                               int subbatch, kron_stream_t stream)
+{
+    if (!out) return KRON_EINVAL;
+    *out = nullptr;
+    kron_xplan_s *p = new kron_xplan_s();
+    p->elem = (int)sizeof(T);
+    KronArgs<T> &a = plan_args<T>(p);
+    kron_status st = xbatched_args(a, d, n, Aarray, lda, pX, pY, pW, batch, subbatch, stream);
+    if (st != KRON_OK) {
+        delete p;
+        return st;
+    }
+    (void)hipGetDevice(&p->device);
+    if (batch > 0 && kron_groupable(a)) {
+        const size_t bytes = group_workspace_bytes(batch);
+        hipError_t e = hipMalloc(&p->ws, bytes);
+        if (e != hipSuccess) {
+            set_last_hip_error((int)e);
+            delete p;
+            return KRON_EHIP;
+        }
+        st = group_by_target(reinterpret_cast<const void *const *>(pY), batch, p->ws, bytes,
+                             &p->ord.keys, &p->ord.perm, static_cast<hipStream_t>(stream));
+        if (st != KRON_OK) {
+            (void)hipFree(p->ws);
+            delete p;
+            return st;
+        }
+        p->grouped = true;
+    }
+    *out = p;
+    return KRON_OK;
+}
+
+template <typename T>
+static kron_status plan_execute(kron_xplan p, kron_stream_t stream)
+{
+    KronArgs<T> a = plan_args<T>(p);
+    if (a.batch == 0) return KRON_OK;
+    a.stream = static_cast<hipStream_t>(stream);
+    a.order = p->grouped ? &p->ord : nullptr;
+    return launch_kron(a);
+}
+
 extern "C" {
 
 const char *kron_status_string(kron_status s)
@@ -274,6 +349,42 @@ kron_status kron_vbatched_f32(int d, const int *m, const int *n, const float *co
                               size_t Wcap, int batch, kron_stream_t s)
 {
     return vbatched_impl<float>(d, m, n, A, pX, pY, W, Wcap, batch, s);
+}
+
+kron_status kron_xbatched_plan_create_f64(kron_xplan *plan, int d, int n,
+                                          const double *const *A, int lda,
+                                          const double *const *pX, double *const *pY,
+                                          double *const *pW, int batch, int subbatch,
+                                          kron_stream_t s)
+{
+    return plan_create<double>(plan, d, n, A, lda, pX, pY, pW, batch, subbatch, s);
+}
+kron_status kron_xbatched_plan_create_f32(kron_xplan *plan, int d, int n,
+                                          const float *const *A, int lda,
+                                          const float *const *pX, float *const *pY,
+                                          float *const *pW, int batch, int subbatch,
+                                          kron_stream_t s)
+{
+    return plan_create<float>(plan, d, n, A, lda, pX, pY, pW, batch, subbatch, s);
+}
+kron_status kron_xbatched_plan_execute(kron_xplan plan, kron_stream_t s)
+{
+    if (!plan) return KRON_EINVAL;
+    return plan->elem == 8 ? plan_execute<double>(plan, s) : plan_execute<float>(plan, s);
+}
+kron_status kron_xbatched_plan_destroy(kron_xplan plan)
+{
+    if (!plan) return KRON_OK;
+    if (plan->ws) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(plan->device);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(plan->ws);
+        (void)hipSetDevice(cur);
+    }
+    delete plan;
+    return KRON_OK;
 }
 
 #define KGEMM_C(NAME, T, TRANS)                                                            \

@@ -120,14 +120,8 @@ constexpr bool fast_pipelined()
     return D >= 2 && (size_t)X::IT * N * sizeof(T) <= 128;
 }
 
-// Entry order for grouped calls: perm[r] is the r-th entry in Y-target order,
-// keys[r] its target (pY[perm[r]]); workgroup w takes positions
-// [w*chunk, (w+1)*chunk).
-struct GroupOrder {
-    const int *perm;
-    const unsigned long long *keys;
-    int chunk;
-};
+// Grouped calls (GroupOrder, kron_internal.hpp): workgroup w takes the
+// positions [w*chunk, (w+1)*chunk) of the Y-target order.
 
 // Grouping needs one entry per workgroup and the mode-1 outputs of a thread
 // (the running Y sum) within 32 VGPRs.

@@ -278,8 +278,12 @@ constexpr int kGroupMinBatch = 2048;
 template <typename T>
 static bool group_order(const KronArgs<T> &a, int N, GroupOrder &ord, hipStream_t s)
 {
-    if (!a.y_shared || a.batch < kGroupMinBatch || a.trans || !fast_groupable_rt<T>(a.d, N))
-        return false;
+    if (!a.y_shared || a.trans || !fast_groupable_rt<T>(a.d, N)) return false;
+    if (a.order) {
+        ord = *a.order;
+        return true;
+    }
+    if (a.batch < kGroupMinBatch) return false;
     const size_t bytes = group_workspace_bytes(a.batch);
     void *ws = stream_workspace(s, bytes);
     if (!ws) return false;
@@ -321,6 +325,20 @@ static kron_status fast_dispatch(const KronArgs<float> &a, int N, hipStream_t s)
     GroupOrder ord{nullptr, nullptr, 0};
     const bool grouped = group_order(a, N, ord, s);
     return fast_ptr_f32(a.d, N, e, a.lda > 0 ? a.lda : N, a.batch, s, grouped ? &ord : nullptr);
+}
+
+template <typename T>
+bool kron_groupable(const KronArgs<T> &a)
+{
+    bool square = true;
+    for (int k = 0; k < a.d; ++k) square = square && (a.m[k] == a.n[k]) && a.m[k] == a.m[0];
+    if (!square || a.layout != Layout::Ptr || !a.y_shared || a.trans) return false;
+    const int N = a.n[0];
+    if (N > fast_max_n<T>(a.d)) return false;
+    long long E = 1;
+    for (int k = 0; k < a.d; ++k) E *= N;
+    if ((long long)sizeof(T) * E > max_lds_bytes()) return false;
+    return fast_groupable_rt<T>(a.d, N);
 }
 
 template <typename T>
@@ -387,6 +405,8 @@ kron_status launch_kron(const KronArgs<T> &a)
 template kron_status launch_kron<double>(const KronArgs<double> &);
 template kron_status launch_kron<float>(const KronArgs<float> &);
 template size_t kron_scratch_elems<double>(const KronArgs<double> &);
+template bool kron_groupable<double>(const KronArgs<double> &);
+template bool kron_groupable<float>(const KronArgs<float> &);
 template size_t kron_scratch_elems<float>(const KronArgs<float> &);
 
 }  // namespace kron

@@ -27,6 +27,14 @@ enum class Scratch : int {
     Linear = 4     // Wbase + slot*slot_stride (vbatched; kronmult_vbatched.hpp:184-185)
 };
 
+// Entry order for merged (grouped) pointer-layout calls: perm[r] is the r-th
+// entry in Y-target order, keys[r] = pY[perm[r]]; chunk is set by the launcher.
+struct GroupOrder {
+    const int *perm;
+    const unsigned long long *keys;
+    int chunk;
+};
+
 template <typename T>
 struct KronArgs {
     int d = 0;
@@ -52,6 +60,8 @@ struct KronArgs {
     bool y_shared = true;    // entries may share Y -> atomic adds
     int batch = 0;
     hipStream_t stream = nullptr;
+    // pre-computed Y-target order (an xbatched plan); nullptr -> decide per call
+    const GroupOrder *order = nullptr;
 };
 
 // Launch the forward kernel family.  Returns KRON_OK, KRON_ENOTSUP when no
@@ -59,6 +69,10 @@ struct KronArgs {
 // or KRON_EHIP on a launch failure.
 template <typename T>
 kron_status launch_kron(const KronArgs<T> &a);
+
+// Whether launch_kron merges this call's entries by Y target (given an order).
+template <typename T>
+bool kron_groupable(const KronArgs<T> &a);
 
 // Scratch elements per entry that launch_kron needs for this shape (0 when the
 // shape runs entirely on chip).

@@ -381,3 +381,22 @@ def test_xbatched_repeat_calls_accumulate():
         K.kronmult6_xbatched(n, pA, n, pX, pY, pX, batch)
     want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, np.zeros((ny, n ** d)), y_map)
     assert np.array_equal(host(Yd), 2 * want)
+
+
+@pytest.mark.parametrize("d,n,batch,ny", [(6, 4, 3000, 7), (6, 4, 40, 3), (4, 5, 2500, 5),
+                                          (6, 6, 5, 2)])
+def test_xbatched_plan(d, n, batch, ny):
+    """kron_xbatched_plan_*: order computed once, executed twice; X changed
+    between executes (allowed), pointer arrays unchanged."""
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 50, ny, n, 33, kind="int")
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(np.zeros((ny, n ** d)))
+    Wd = torch.empty((max(2, batch), n ** d), dtype=torch.float64, device=DEV)
+    pA, pX, pY = ptrs(Ad, np.arange(batch * d)), ptrs(Xd, x_map), ptrs(Yd, y_map)
+    pW = ptrs(Wd, np.arange(max(2, batch)))
+    plan = K.XbatchedPlan(d, n, pA, n, pX, pY, pW, batch)
+    plan.execute()
+    Xd.mul_(2.0)
+    plan.execute()
+    plan.close()
+    want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, np.zeros((ny, n ** d)), y_map)
+    assert np.array_equal(host(Yd), 3 * want)
