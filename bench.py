@@ -211,6 +211,37 @@ def bench_batched(args):
                       "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}), flush=True)
 
 
+def bench_adjoint(args):
+    """Forward and hand-written adjoint, FP64, d=6, n=6, 4,096 entries
+    (batched layout; A, X, dY uniform [-1,1])."""
+    import torch
+
+    import kronmult_amd as K
+
+    d, n, batch = 6, 6, 4096
+    E = n ** d
+    dev = "cuda:0"
+    A = torch.from_numpy(_splitmix_uniform(batch * d * n * n, 31)).to(dev)
+    X = torch.from_numpy(_splitmix_uniform(batch * E, 32)).to(dev)
+    dY = torch.from_numpy(_splitmix_uniform(batch * E, 33)).to(dev)
+    Y = torch.zeros(batch * E, dtype=torch.float64, device=dev)
+    W = torch.empty(2 * batch * E, dtype=torch.float64, device=dev)
+    dX = torch.empty_like(X)
+    dA = torch.empty_like(A)
+    stream = torch.cuda.current_stream()
+    ms_f = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, W, batch, stream=stream),
+                       args.steps, args.warmup, stream)
+    ms_a = _time_steps(lambda: K.kronmult_batched_adjoint(d, n, A, X, dY, dX, dA, W, batch,
+                                                          stream=stream),
+                       args.steps, args.warmup, stream)
+    fwd_bytes = bytes_per_entry(d, n, ptr_layout=False) * batch
+    adj_bytes = 8 * (3 * E + 2 * d * n * n) * batch  # X, dY read, dX written, A read, dA written
+    print(json.dumps({"workload": "forward + adjoint d=6 n=6 batchCount=4096", "dtype": "f64",
+                      "forward_ms": ms_f, "adjoint_ms": ms_a, "adjoint_over_forward": ms_a / ms_f,
+                      "forward_alg_gbs": fwd_bytes / (ms_f * 1e-3) / 1e9,
+                      "adjoint_alg_gbs": adj_bytes / (ms_a * 1e-3) / 1e9}), flush=True)
+
+
 def bench_copy(args):
     """Device-to-device copy of 2 GiB: the achievable HBM read+write rate."""
     import torch
@@ -241,6 +272,8 @@ def main():
         bench_xbatched(args)
     elif args.workload == "batched_d6n4":
         bench_batched(args)
+    elif args.workload == "adjoint_d6n6":
+        bench_adjoint(args)
     elif args.workload == "copy":
         bench_copy(args)
     else:

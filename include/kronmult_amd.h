@@ -65,6 +65,22 @@ kron_status kron_batched_f32(int d, int n, const float *A, const float *X,
                              kron_stream_t stream);
 
 /*
+ * Adjoint of the batched layout (no counterpart in the reference).  With
+ * K = kron(A1, ..., Ad) per entry and an upstream gradient dY (n^d, batch):
+ *   dX = K^T dY                      (n^d, batch), overwritten
+ *   dA(:,:,k,b) = d<dY, K X>/dA_k    (n, n, d, batch), overwritten
+ * Intermediates are recomputed on chip; only two vectors per in-flight entry
+ * go through W, which must hold 2*n^d*batchCount elements (W may be NULL
+ * when the whole entry fits on chip; KRON_EWORKSPACE otherwise).
+ */
+kron_status kron_batched_adjoint_f64(int d, int n, const double *A, const double *X,
+                                     const double *dY, double *dX, double *dA, double *W,
+                                     int batchCount, kron_stream_t stream);
+kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *X,
+                                     const float *dY, float *dX, float *dA, float *W,
+                                     int batchCount, kron_stream_t stream);
+
+/*
  * Pointer-array layout with shared Y.  Replaces kronmult{1..6}_xbatched
  * (kronmult6_xbatched.cpp:5-39, body kronmult_xbatched.hpp:19-157).
  *   Aarray[k + b*d] -> n x n factor k of entry b, leading dimension lda >= n

@@ -21,7 +21,7 @@ import os
 
 __all__ = [
     "KronError", "lib", "ptr_array", "XbatchedPlan",
-    "kronmult_batched", "kronmult_xbatched", "kronmult_vbatched",
+    "kronmult_batched", "kronmult_batched_adjoint", "kronmult_xbatched", "kronmult_vbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
 ] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
 
@@ -46,6 +46,8 @@ _SIGS = {
     "kron_batched_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp],
     "kron_xbatched_%s": [_c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_vp],
     "kron_vbatched_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int, _c_vp],
+    "kron_batched_adjoint_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int,
+                                _c_vp],
 }
 
 
@@ -133,6 +135,19 @@ def kronmult_batched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, stream=No
     _check("kronmult%d_batched" % d,
            f(d, n, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(Warray), batchCount,
              _stream(stream)))
+
+
+def kronmult_batched_adjoint(d, n, Aarray, Xarray, dYarray, dXarray, dAarray, Warray,
+                             batchCount, stream=None):
+    """Adjoint of kronmult{d}_batched (no counterpart in the reference):
+    dX = kron(A1..Ad)^T dY and dA = d<dY, kron(A) X>/dA, both overwritten.
+    Warray: 2 * n^d * batchCount elements of scratch, or None when the entry
+    fits on chip."""
+    sfx = _sfx(Aarray.dtype)
+    f = getattr(lib(), "kron_batched_adjoint_" + sfx)
+    _check("kronmult%d_batched_adjoint" % d,
+           f(d, n, _addr(Aarray), _addr(Xarray), _addr(dYarray), _addr(dXarray),
+             _addr(dAarray), _addr(Warray), batchCount, _stream(stream)))
 
 
 def kronmult_xbatched(d, n, Aarray, lda, Xarray, Yarray, Warray, batchCount,

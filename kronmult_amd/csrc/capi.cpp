@@ -217,6 +217,20 @@ static kron_status vbatched_impl(int d, const int *m, const int *n, const T *con
 }
 
 template <typename T>
+static kron_status adjoint_impl(int d, int n, const T *A, const T *X, const T *dY, T *dX, T *dA,
+                                T *W, int batch, kron_stream_t stream)
+{
+    long long E = 0;
+    if (d < 1 || d > kMaxDim || n < 1 || batch < 0 || !ipow_ok(n, d, E)) return KRON_EINVAL;
+    if (batch == 0) return KRON_OK;
+    if (!A || !X || !dY || !dX || !dA) return KRON_EINVAL;
+    const size_t need = adjoint_scratch_elems<T>(d, n);
+    if (need == (size_t)-1) return KRON_ENOTSUP;
+    if (need > 0 && !W) return KRON_EWORKSPACE;
+    return launch_adjoint<T>(d, n, A, X, dY, dX, dA, W, batch, static_cast<hipStream_t>(stream));
+}
+
+template <typename T>
 static kron_status kgemm_impl(bool transB, int mm, int nn, int kk, T alpha,
                               const T *const *A, const int *ldA, const T *const *B,
                               const int *ldB, T beta, T *const *C, const int *ldC, int batch,
@@ -325,6 +339,18 @@ kron_status kron_batched_f32(int d, int n, const float *A, const float *X, float
                              int batch, kron_stream_t s)
 {
     return batched_impl<float>(d, n, A, X, Y, W, batch, s);
+}
+kron_status kron_batched_adjoint_f64(int d, int n, const double *A, const double *X,
+                                     const double *dY, double *dX, double *dA, double *W,
+                                     int batch, kron_stream_t s)
+{
+    return adjoint_impl<double>(d, n, A, X, dY, dX, dA, W, batch, s);
+}
+kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *X,
+                                     const float *dY, float *dX, float *dA, float *W, int batch,
+                                     kron_stream_t s)
+{
+    return adjoint_impl<float>(d, n, A, X, dY, dX, dA, W, batch, s);
 }
 kron_status kron_xbatched_f64(int d, int n, const double *const *A, int lda,
                               const double *const *pX, double *const *pY, double *const *pW,

@@ -186,6 +186,109 @@ __device__ __forceinline__ void y_accumulate(P y, long long idx, Acc v)
     }
 }
 
+// dst = op(A) x_q src for mode q of an LDS tensor with extents ext[0..R):
+// ext[q] = n on input, m on output; A is m x n (n x m stored when TRANS) in
+// LDS with leading dimension ldA.  One thread per fiber: n loads into
+// registers (n <= NMAX), m outputs; consecutive threads take consecutive
+// fast indices.  Larger n falls back to one thread per output element.
+template <typename T, typename Acc, bool TRANS, int NMAX>
+__device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__restrict__ dst,
+                                                const int *ext, int R, int q, int m,
+                                                const T *__restrict__ Al, int ldA)
+{
+    int inner = 1;
+    for (int r = q + 1; r < R; ++r) inner *= ext[r];
+    int outer = 1;
+    for (int r = 0; r < q; ++r) outer *= ext[r];
+    const int n = ext[q];
+    if (n <= NMAX) {
+        const int fibers = outer * inner;
+        for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
+            const int ii = f % inner, o = f / inner;
+            const T *s = src + (o * n) * inner + ii;
+            T *dd = dst + (o * m) * inner + ii;
+            Acc in[NMAX];
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j) in[j] = (j < n) ? Acc(s[j * inner]) : Acc(0);
+            for (int i = 0; i < m; ++i) {
+                Acc acc = 0;
+#pragma unroll
+                for (int j = 0; j < NMAX; ++j)
+                    if (j < n)
+                        acc = __builtin_fma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
+                                            in[j], acc);
+                dd[i * inner] = T(acc);
+            }
+        }
+    } else {
+        const int total = outer * m * inner;
+        for (int w = threadIdx.x; w < total; w += blockDim.x) {
+            const int ii = w % inner;
+            const int rest = w / inner;
+            const int i = rest % m;
+            const int o = rest / m;
+            const T *s = src + (o * n) * inner + ii;
+            Acc acc = 0;
+            for (int j = 0; j < n; ++j)
+                acc = __builtin_fma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
+                                    Acc(s[j * inner]), acc);
+            dst[w] = T(acc);
+        }
+    }
+}
+
+// Square variant with compile-time N: the factor is read once from LDS into
+// registers (N*N values) and each thread contracts whole fibers, N loads and
+// N*N FMAs each.  SQ == 0 selects contract_fibers (runtime extents).
+template <typename T, typename Acc, bool TRANS, int SQ>
+__device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *__restrict__ dst,
+                                                  const int *ext, int R, int q, int m,
+                                                  const T *__restrict__ Al, int ldA)
+{
+    if constexpr (SQ == 0) {
+        contract_fibers<T, Acc, TRANS, 8>(src, dst, ext, R, q, m, Al, ldA);
+    } else {
+        constexpr int N = SQ;
+        int inner = 1;
+        for (int r = q + 1; r < R; ++r) inner *= ext[r];
+        int outer = 1;
+        for (int r = 0; r < q; ++r) outer *= ext[r];
+        // the factor is the same for the whole workgroup: up to N = 6 it is kept
+        // in SGPRs (v_fma_f64 takes one scalar operand), above in VGPRs
+        Acc a[N][N];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const Acc v = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
+                a[i][j] = (N <= 6) ? uniform_val(v) : v;
+            }
+        const int fibers = outer * inner;
+        for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
+            const int ii = f % inner, o = f / inner;
+            const int base = (o * N) * inner + ii;
+            Acc in[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) in[j] = Acc(src[base + j * inner]);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                Acc acc = a[i][0] * in[0];
+#pragma unroll
+                for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
+                dst[base + i * inner] = T(acc);
+            }
+        }
+    }
+}
+
+// Copy an m x n column-major factor (leading dimension ld) into LDS, packed
+// with leading dimension m.
+template <typename T, typename PtrT>
+__device__ __forceinline__ void stage_factor(T *dst, PtrT src, int m, int n, int ld)
+{
+    for (int w = threadIdx.x; w < m * n; w += blockDim.x) dst[w] = src[(w % m) + (w / m) * ld];
+}
+
 // Compile-time loop K = HI, HI-1, ..., LO calling f(integral_constant<K>).
 template <int HI, int LO, typename F>
 __device__ __forceinline__ void for_modes_desc(F &&f)

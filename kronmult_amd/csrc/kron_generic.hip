@@ -35,6 +35,8 @@ struct GenShape {
     int chunk1;             // low indices per pass-1 chunk
     int chunk2;             // slabs per pass-2 chunk
     int lds_elems;          // elements per LDS ping-pong buffer
+    int fac_off[kMaxDim];   // LDS offset of each staged factor (elements)
+    int fac_elems;          // LDS elements of all staged factors
     long long nlo_in;       // prod_{k>=h} n_k
     long long nhi_in;       // prod_{k<h}  n_k
     long long nhi_out;      // prod_{k<h}  m_k
@@ -61,107 +63,168 @@ struct ScratchDesc {
     }
 };
 
-// dst(o, i, ii) = sum_j A(i, j) * src(o, j, ii) for mode q of a tensor with
-// extents ext[0..R), ext[q] = n on input, m on output.  Output index == item.
-template <typename T, typename Acc, bool TRANS>
-__device__ __forceinline__ void contract_mode(const T *__restrict__ src, T *__restrict__ dst,
-                                              const int *ext, int R, int q, int m,
-                                              const T *__restrict__ Ap, int ld)
-{
-    int inner = 1;
-    for (int r = q + 1; r < R; ++r) inner *= ext[r];
-    int outer = 1;
-    for (int r = 0; r < q; ++r) outer *= ext[r];
-    const int n = ext[q];
-    const int total = outer * m * inner;
-    for (int w = threadIdx.x; w < total; w += blockDim.x) {
-        const int ii = w % inner;
-        const int rest = w / inner;
-        const int i = rest % m;
-        const int o = rest / m;
-        const T *s = src + (o * n) * inner + ii;
-        Acc acc = 0;
-        for (int j = 0; j < n; ++j) {
-            const Acc aij = TRANS ? Acc(Ap[j + i * ld]) : Acc(Ap[i + j * ld]);
-            acc = __builtin_fma(aij, Acc(s[j * inner]), acc);
+// Elements of one work item's input held per thread while the previous item
+// is contracted (the item's LDS chunk is at most kStagePer * 256 elements).
+constexpr int kStagePer = 8;
+
+// A work item is one chunk of pass 1 (slow modes over the consecutive low
+// indices [c0, c0+C)) or of pass 2 (fast modes over the slabs [s0, s0+C)) of
+// one entry.  Its input is first loaded into registers (RegStage), so the
+// loads of item r+1 are in flight while item r is contracted in LDS.
+template <typename T>
+struct RegStage {
+    T v[kStagePer];
+};
+
+template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS, int SQ>
+struct GenericPasses {
+    const Entries<T, LAY> &ent;
+    const GenShape &sh;
+    const ScratchDesc<T> &scr;
+    int phase, b0;
+    long long nck;
+    T *buf0, *buf1, *facs;
+
+    __device__ __forceinline__ void decode(long long r, int &b, long long &c0, int &C) const
+    {
+        b = b0 + (int)(r / nck);
+        const long long ck = r % nck;
+        if (phase == 1) {
+            c0 = ck * sh.chunk1;
+            C = (int)min((long long)sh.chunk1, sh.nlo_in - c0);
+        } else {
+            c0 = ck * sh.chunk2;
+            C = (int)min((long long)sh.chunk2, sh.nhi_out - c0);
         }
-        dst[w] = T(acc);
     }
-}
-
-template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS>
-__global__ void __launch_bounds__(256)
-kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int batch)
-{
-    extern __shared__ __align__(16) unsigned char kron_smem[];
-    T *const buf0 = reinterpret_cast<T *>(kron_smem);
-    T *const buf1 = buf0 + sh.lds_elems;
-    const int tid = threadIdx.x;
-    const int nthr = blockDim.x;
-    const int d = sh.d;
-    const int h = sh.h;
-
-    for (int b = blockIdx.x; b < batch; b += gridDim.x) {
-        const T *__restrict__ x = ent.x(b);
-        T *y = ent.y(b);
-        T *S = (h > 0) ? scr.get(blockIdx.x, b) : nullptr;
-        int ext[kMaxDim + 1];
-
-        if (h > 0) {
-            // pass 1: slow modes 0..h-1 over chunks of low indices
-            for (long long c0 = 0; c0 < sh.nlo_in; c0 += sh.chunk1) {
-                const int C = (int)min((long long)sh.chunk1, sh.nlo_in - c0);
-                const int nin = (int)sh.nhi_in * C;
-                for (int w = tid; w < nin; w += nthr) {
+    __device__ __forceinline__ const T *src_of(int b) const
+    {
+        if (phase == 1 || sh.h == 0) return ent.x(b);
+        return scr.get(b - b0, b);
+    }
+    __device__ __forceinline__ int count(int C) const
+    {
+        return (phase == 1) ? (int)sh.nhi_in * C : C * (int)sh.nlo_in;
+    }
+    __device__ __forceinline__ void load(RegStage<T> &st, long long r) const
+    {
+        int b, C;
+        long long c0;
+        decode(r, b, c0, C);
+        const auto src = to_global(src_of(b));
+        const int cnt = count(C);
+#pragma unroll
+        for (int k = 0; k < kStagePer; ++k) {
+            const int w = threadIdx.x + k * 256;
+            if (w < cnt) {
+                if (phase == 1) {
                     const int hi = w / C, c = w % C;
-                    buf0[w] = x[(long long)hi * sh.nlo_in + c0 + c];
+                    st.v[k] = src[(long long)hi * sh.nlo_in + c0 + c];
+                } else {
+                    st.v[k] = src[c0 * sh.nlo_in + w];
                 }
-                __syncthreads();
-                for (int k = 0; k < h; ++k) ext[k] = sh.n[k];
-                ext[h] = C;
-                T *src = buf0, *dst = buf1;
-                for (int q = 0; q < h; ++q) {
-                    contract_mode<T, Acc, TRANS>(src, dst, ext, h + 1, q, sh.m[q],
-                                                 ent.factor(b, q), sh.ld[q]);
-                    ext[q] = sh.m[q];
-                    __syncthreads();
-                    T *t = src; src = dst; dst = t;
-                }
-                const int nout = (int)sh.nhi_out * C;
-                for (int w = tid; w < nout; w += nthr) {
-                    const int hi = w / C, c = w % C;
-                    S[(long long)hi * sh.nlo_in + c0 + c] = src[w];
-                }
-                __syncthreads();
             }
         }
-
-        // pass 2 (or the only pass): fast modes h..d-1 over slabs
-        const T *__restrict__ src2 = (h > 0) ? S : x;
-        for (long long s0 = 0; s0 < sh.nhi_out; s0 += sh.chunk2) {
-            const int C2 = (int)min((long long)sh.chunk2, sh.nhi_out - s0);
-            const int nin = C2 * (int)sh.nlo_in;
-            const T *srcg = src2 + s0 * sh.nlo_in;
-            for (int w = tid; w < nin; w += nthr) buf0[w] = srcg[w];
-            __syncthreads();
-            ext[0] = C2;
+    }
+    __device__ __forceinline__ void to_lds(const RegStage<T> &st, int cnt) const
+    {
+#pragma unroll
+        for (int k = 0; k < kStagePer; ++k) {
+            const int w = threadIdx.x + k * 256;
+            if (w < cnt) buf0[w] = st.v[k];
+        }
+    }
+    __device__ __forceinline__ void stage_factors(int b) const
+    {
+        for (int k = 0; k < sh.d; ++k) {
+            const int rows = TRANS ? sh.n[k] : sh.m[k], cols = TRANS ? sh.m[k] : sh.n[k];
+            stage_factor(facs + sh.fac_off[k], ent.template factor_g<true>(b, k), rows, cols,
+                         sh.ld[k]);
+        }
+    }
+    // contract the item's modes in LDS (input in buf0) and store the result
+    __device__ __forceinline__ void compute_store(int b, long long c0, int C) const
+    {
+        const int tid = threadIdx.x, nthr = blockDim.x, h = sh.h, d = sh.d;
+        int ext[kMaxDim + 1];
+        T *src = buf0, *dst = buf1;
+        if (phase == 1) {
+            for (int k = 0; k < h; ++k) ext[k] = sh.n[k];
+            ext[h] = C;
+            for (int q = 0; q < h; ++q) {
+                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, ext, h + 1, q, sh.m[q],
+                                                     facs + sh.fac_off[q],
+                                                     TRANS ? sh.n[q] : sh.m[q]);
+                ext[q] = sh.m[q];
+                __syncthreads();
+                T *t = src; src = dst; dst = t;
+            }
+            T *S = scr.get(b - b0, b);
+            const int nout = (int)sh.nhi_out * C;
+            for (int w = tid; w < nout; w += nthr) {
+                const int hi = w / C, c = w % C;
+                S[(long long)hi * sh.nlo_in + c0 + c] = src[w];
+            }
+        } else {
+            ext[0] = C;
             for (int k = h; k < d; ++k) ext[1 + k - h] = sh.n[k];
             const int R = 1 + d - h;
-            T *src = buf0, *dst = buf1;
             for (int k = h; k < d; ++k) {
                 const int q = 1 + k - h;
-                contract_mode<T, Acc, TRANS>(src, dst, ext, R, q, sh.m[k], ent.factor(b, k),
-                                             sh.ld[k]);
+                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, ext, R, q, sh.m[k],
+                                                     facs + sh.fac_off[k],
+                                                     TRANS ? sh.n[k] : sh.m[k]);
                 ext[q] = sh.m[k];
                 __syncthreads();
                 T *t = src; src = dst; dst = t;
             }
-            const int nout = C2 * (int)sh.nlo_out;
-            const long long ybase = s0 * sh.nlo_out;
+            const auto y = to_global(ent.y(b));
+            const int nout = C * (int)sh.nlo_out;
+            const long long ybase = c0 * sh.nlo_out;
             for (int w = tid; w < nout; w += nthr)
                 y_accumulate<SHARED, T, Acc>(y, ybase + w, Acc(src[w]));
-            __syncthreads();
         }
+    }
+};
+
+// One pass (phase 1: slow modes into the scratch, phase 2: fast modes into Y)
+// over the work items (entry, chunk) of the entries [b0, b1).  Each workgroup
+// walks a contiguous range of items, so consecutive items usually belong to
+// the same entry (factors are restaged only when the entry changes); entry b
+// uses scratch slot b - b0.
+template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS, int SQ>
+__global__ void __launch_bounds__(256)
+kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int phase, int b0,
+                    int b1)
+{
+    extern __shared__ __align__(16) unsigned char kron_smem[];
+    T *const buf0 = reinterpret_cast<T *>(kron_smem);
+    T *const buf1 = buf0 + sh.lds_elems;
+    T *const facs = buf1 + sh.lds_elems;  // the entry's factors, packed (ld = rows)
+    const long long nck = (phase == 1) ? (sh.nlo_in + sh.chunk1 - 1) / sh.chunk1
+                                       : (sh.nhi_out + sh.chunk2 - 1) / sh.chunk2;
+    const GenericPasses<T, Acc, LAY, SHARED, TRANS, SQ> P{ent, sh, scr, phase, b0, nck,
+                                                          buf0, buf1, facs};
+    const long long items = (long long)(b1 - b0) * nck;
+    const long long per = (items + gridDim.x - 1) / gridDim.x;
+    const long long r0 = blockIdx.x * per;
+    const long long r1 = min(items, r0 + per);
+    RegStage<T> st;
+    if (r0 < r1) P.load(st, r0);
+    int cur_b = -1;
+    for (long long r = r0; r < r1; ++r) {
+        int b, C;
+        long long c0;
+        P.decode(r, b, c0, C);
+        if (b != cur_b) {
+            P.stage_factors(b);
+            cur_b = b;
+        }
+        P.to_lds(st, P.count(C));
+        __syncthreads();
+        if (r + 1 < r1) P.load(st, r + 1);
+        P.compute_store(b, c0, C);
+        __syncthreads();
     }
 }
 
@@ -206,7 +269,10 @@ static bool plan_generic(GenShape &sh, long long budget)
         long long c1 = std::min(nlo_in, budget / per1);
         long long c2 = std::min(nhi_out, budget / per2);
         if (c1 < 1 || c2 < 1) continue;
-        const long long iters = (nlo_in + c1 - 1) / c1 + (nhi_out + c2 - 1) / c2;
+        // pass 1 reads rows of c1 consecutive elements: short rows waste HBM
+        // bursts, so they count four times
+        const long long iters = ((nlo_in + c1 - 1) / c1) * ((c1 < 16 && c1 < nlo_in) ? 4 : 1) +
+                                (nhi_out + c2 - 1) / c2;
         if (!found || iters < best_iters) {
             found = true;
             best_iters = iters;
@@ -233,26 +299,77 @@ static bool make_generic_shape(const KronArgs<T> &a, GenShape &sh)
         sh.n[k] = a.trans ? a.m[k] : a.n[k];
         sh.ld[k] = (a.lda > 0) ? a.lda : a.m[k];
     }
-    const long long budget = (long long)(max_lds_bytes() / 2) / (long long)sizeof(T);
-    return plan_generic(sh, budget);
+    int off = 0;
+    for (int k = 0; k < kMaxDim; ++k) {
+        sh.fac_off[k] = off;
+        if (k < a.d) off += sh.m[k] * sh.n[k];
+    }
+    sh.fac_elems = (off + 1) & ~1;  // keep the buffers 16-byte aligned for doubles
+    // Ping-pong buffers of at most kStagePer*256 elements (the register stage
+    // of one work item): 2 x 16 KiB for doubles, four workgroups per CU.
+    const long long budget = std::min<long long>(kStagePer * 256,
+                                                 (max_lds_bytes() / 2) / (long long)sizeof(T));
+    if (!plan_generic(sh, budget)) return false;
+    return (2LL * sh.lds_elems + sh.fac_elems) * (long long)sizeof(T) <= max_lds_bytes();
 }
 
-template <typename T, Layout LAY, bool SHARED, bool TRANS>
-static kron_status launch_generic_t(const Entries<T, LAY> &ent, const GenShape &sh,
-                                    const ScratchDesc<T> &scr, int grid, int batch,
-                                    hipStream_t stream)
+template <typename T, Layout LAY, bool SHARED, bool TRANS, int SQ>
+static kron_status launch_generic_sq(const Entries<T, LAY> &ent, const GenShape &sh,
+                                     const ScratchDesc<T> &scr, int group, int batch,
+                                     hipStream_t stream)
 {
-    auto kern = kron_generic_kernel<T, T, LAY, SHARED, TRANS>;
-    const size_t lds = 2 * (size_t)sh.lds_elems * sizeof(T);
+    auto kern = kron_generic_kernel<T, T, LAY, SHARED, TRANS, SQ>;
+    const size_t lds = (2 * (size_t)sh.lds_elems + sh.fac_elems) * sizeof(T);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, stream, ent, sh, scr, batch);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
+    const long long resident = persistent_blocks(reinterpret_cast<const void *>(kern), lds);
+    const long long cap = resident > 0 ? 8 * resident : 65536;
+    auto launch = [&](int phase, int b0, int b1, long long items) -> kron_status {
+        const long long grid = std::max(1LL, std::min(items, cap / 8 > 0 ? cap / 8 : cap));
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, stream, ent, sh, scr, phase,
+                           b0, b1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
+        return KRON_OK;
+    };
+    const long long nck1 = sh.h > 0 ? (sh.nlo_in + sh.chunk1 - 1) / sh.chunk1 : 0;
+    const long long nck2 = (sh.nhi_out + sh.chunk2 - 1) / sh.chunk2;
+    // entries are processed in groups whose scratch vectors are all distinct
+    for (int b0 = 0; b0 < batch; b0 += group) {
+        const int b1 = std::min(batch, b0 + group);
+        kron_status st;
+        if (sh.h > 0) {
+            st = launch(1, b0, b1, (long long)(b1 - b0) * nck1);
+            if (st != KRON_OK) return st;
+        }
+        st = launch(2, b0, b1, (long long)(b1 - b0) * nck2);
+        if (st != KRON_OK) return st;
+    }
     return KRON_OK;
+}
+
+// Square shapes with n <= 8 get the compile-time contraction (factor in
+// registers); everything else the runtime-extent one.
+template <typename T, Layout LAY, bool SHARED, bool TRANS>
+static kron_status launch_generic_t(const Entries<T, LAY> &ent, const GenShape &sh,
+                                    const ScratchDesc<T> &scr, int group, int batch,
+                                    hipStream_t stream)
+{
+    bool square = true;
+    for (int k = 0; k < sh.d; ++k) square = square && sh.m[k] == sh.n[0] && sh.n[k] == sh.n[0];
+    const int N = square ? sh.n[0] : 0;
+    switch (N) {
+#define KRON_SQ(NN)                                                                          \
+    case NN:                                                                                 \
+        return launch_generic_sq<T, LAY, SHARED, TRANS, NN>(ent, sh, scr, group, batch, stream);
+        KRON_SQ(2) KRON_SQ(3) KRON_SQ(4) KRON_SQ(5) KRON_SQ(6) KRON_SQ(7) KRON_SQ(8)
+#undef KRON_SQ
+    default:
+        return launch_generic_sq<T, LAY, SHARED, TRANS, 0>(ent, sh, scr, group, batch, stream);
+    }
 }
 
 template <typename T>
@@ -356,7 +473,7 @@ kron_status launch_kron(const KronArgs<T> &a)
     GenShape sh{};
     if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
-    int grid = a.batch;
+    int grid = a.batch;  // entries whose scratch vectors are distinct (one launch group)
     if (sh.h > 0) {
         scr.kind = a.scratch;
         switch (a.scratch) {

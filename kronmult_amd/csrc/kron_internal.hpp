@@ -96,6 +96,15 @@ kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t w
 // is ordered, so calls on the same stream may reuse it.  nullptr on failure.
 void *stream_workspace(hipStream_t stream, size_t bytes);
 
+// Adjoint of the batched layout (kron_adjoint.hip): dX = K^T dY and dA_i,
+// both overwritten.  W: 2*n^d elements per entry (or nullptr when
+// adjoint_scratch_elems returns 0).
+template <typename T>
+kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T *dX, T *dA, T *W,
+                           int batch, hipStream_t stream);
+template <typename T>
+size_t adjoint_scratch_elems(int d, int n);
+
 // Per-thread record of the last HIP error (kron_last_hip_error()).
 void set_last_hip_error(int code);
 int max_lds_bytes();
