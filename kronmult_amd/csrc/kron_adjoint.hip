@@ -35,45 +35,63 @@ struct AdjShape {
     int nbuf;              // LDS buffers
 };
 
-// dA(p, r) += sum_{o, ii} G[o, p, ii] * P[o, r, ii] for mode q; dA in LDS
-// (column-major n x n, double accumulation, LDS atomics once per thread).
+// n^k for the small uniform exponents of an entry's mode extents.
+__device__ __forceinline__ int ipow(int n, int k)
+{
+    int p = 1;
+    for (int i = 0; i < k; ++i) p *= n;
+    return p;
+}
+
+// dA(p, r) += sum_{o, ii} G[o, p, ii] * P[o, r, ii] over tensors viewed as
+// outer x n x inner (dA in LDS, column-major n x n, double accumulation).
+// Thread t takes the pair t % npair and every (t / npair)-th (o, ii)
+// position, so the lanes of a wave read a handful of broadcast addresses;
+// indices advance without division.
 template <typename T>
 __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__restrict__ P,
-                                         const int *ext, int R, int q, double *dA)
+                                         int outer, int n, int inner, double *dA)
 {
-    int inner = 1;
-    for (int r = q + 1; r < R; ++r) inner *= ext[r];
-    int outer = 1;
-    for (int r = 0; r < q; ++r) outer *= ext[r];
-    const int n = ext[q];
     const int npair = n * n;
     const int rest = outer * inner;
     const int nthr = blockDim.x;
     for (int pair0 = 0; pair0 < npair; pair0 += nthr) {
         const int pairs = min(nthr, npair - pair0);
-        const int per = nthr / pairs;  // threads per (p, r) pair
+        const int per = nthr / pairs;  // threads sharing a pair
         const int t = threadIdx.x;
         if (t < pairs * per) {
-            const int pair = pair0 + t / per;
-            const int sub = t % per;
+            const int pair = pair0 + t % pairs;
+            const int sub = t / pairs;
             const int p = pair % n, r = pair / n;
+            const int d_o = per / inner, d_ii = per % inner;
+            int o = sub / inner, ii = sub % inner;
             double s = 0;
             for (int k = sub; k < rest; k += per) {
-                const int o = k / inner, ii = k % inner;
                 const int base = (o * n) * inner + ii;
                 s = __builtin_fma(double(G[base + p * inner]), double(P[base + r * inner]), s);
+                o += d_o;
+                ii += d_ii;
+                if (ii >= inner) {
+                    ii -= inner;
+                    ++o;
+                }
             }
             unsafeAtomicAdd(&dA[pair], s);
         }
     }
 }
 
+// Elements of one work item's input held per thread per input stream while
+// the previous item is worked on (an LDS buffer is at most kAdjPer*256).
+constexpr int kAdjPer = 8;
+
 // One pass of the adjoint over work items (entry, chunk):
-//   phase 1: slab chunk  -> V = (I_S (x) A_F^T) dY
+//   phase 1: slab chunk   -> V = (I_S (x) A_F^T) dY
 //   phase 2: column chunk -> U, dX, dA_S partial sums
-//   phase 3: slab chunk  -> dA_F partial sums
-// Partial dA sums of a work item are accumulated in LDS and added to dA (zeroed
-// beforehand) with one atomic per element.
+//   phase 3: slab chunk   -> dA_F partial sums
+// Each workgroup walks a contiguous range of items with the next item's
+// inputs prefetched into registers; dA partial sums stay in LDS while the
+// entry does not change and are then added to dA (cleared beforehand).
 template <typename T, typename Acc, int SQ>
 __global__ void __launch_bounds__(256)
 kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *__restrict__ dY,
@@ -90,122 +108,157 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long long E = sh.E, ES = sh.ES, EF = sh.EF;
     const int nf = d - h;  // number of fast modes
-    int ext[kMaxDim + 1];
+    const int ddn = d * n * n;
     const long long nck = (phase == 2) ? (EF + sh.C2 - 1) / sh.C2 : (ES + sh.C1 - 1) / sh.C1;
     const long long items = (long long)batch * nck;
+    const long long per = (items + gridDim.x - 1) / gridDim.x;
+    const long long r0 = blockIdx.x * per, r1 = min(items, r0 + per);
 
-    for (long long it = blockIdx.x; it < items; it += gridDim.x) {
-        const int b = (int)(it / nck);
-        const long long ck = it % nck;
-        const T *Ab = A + (long long)b * d * n * n;
-        for (int w = tid; w < d * n * n; w += nthr) {
-            facL[w] = Ab[w];
-            dAacc[w] = 0;
+    // item r -> entry, chunk start, chunk size, element count
+    auto decode = [&](long long r, int &b, long long &s0, int &C, int &cnt)
+                      __attribute__((always_inline)) {
+        b = (int)(r / nck);
+        const long long ck = r % nck;
+        if (phase == 2) {
+            s0 = ck * sh.C2;
+            C = (int)min((long long)sh.C2, EF - s0);
+            cnt = (int)ES * C;
+        } else {
+            s0 = ck * sh.C1;
+            C = (int)min((long long)sh.C1, ES - s0);
+            cnt = C * (int)EF;
         }
-        const T *x = X + b * E;
+    };
+    T v0[kAdjPer], v1[kAdjPer];
+    auto load = [&](long long r) __attribute__((always_inline)) {
+        int b, C, cnt;
+        long long s0;
+        decode(r, b, s0, C, cnt);
+        // phase 1: dY slab; phase 2: X and V (dY when every mode is slow)
+        // columns; phase 3: U and dY slabs
         const T *dy = dY + b * E;
-        T *V = (nf > 0) ? W + (long long)b * 2 * E : nullptr;
-        T *U = (nf > 0) ? V + E : nullptr;
+        const T *V = W + (long long)b * 2 * E;
+        const T *p0 = (phase == 1) ? dy : (phase == 2) ? X + b * E : V + E;
+        const T *p1 = (phase == 2 && nf > 0) ? V : dy;
+        if (phase != 2) {
+            p0 += s0 * EF;
+            p1 += s0 * EF;
+        }
+        const auto g0 = to_global(p0), g1 = to_global(p1);
+#pragma unroll
+        for (int k = 0; k < kAdjPer; ++k) {
+            const int w = tid + k * 256;
+            if (w < cnt) {
+                const long long g = (phase == 2) ? (long long)(w / C) * EF + s0 + w % C : w;
+                v0[k] = g0[g];
+                if (phase != 1) v1[k] = g1[g];
+            }
+        }
+    };
+    auto flush = [&](int b) __attribute__((always_inline)) {
+        // this entry's share of dA: modes [0, h) in phase 2, [h, d) in phase 3
+        const int k0 = (phase == 2) ? 0 : h, k1 = (phase == 2) ? h : d;
+        const auto dAb = to_global(dA + (long long)b * ddn);
+        for (int w = k0 * n * n + tid; w < k1 * n * n; w += nthr) atomic_add(dAb + w, T(dAacc[w]));
+    };
+
+    for (int w = tid; w < ddn; w += nthr) dAacc[w] = 0;
+    if (r0 < r1) load(r0);
+    int cur_b = -1;
+    for (long long r = r0; r < r1; ++r) {
+        int b, C, cnt;
+        long long s0;
+        decode(r, b, s0, C, cnt);
+        if (b != cur_b) {
+            if (cur_b >= 0 && phase != 1) flush(cur_b);
+            for (int w = tid; w < ddn; w += nthr) {
+                dAacc[w] = 0;
+                facL[w] = A[(long long)b * ddn + w];
+            }
+            cur_b = b;
+        }
+        // the prefetched inputs into LDS
+        const int i0 = (phase == 1) ? 0 : (phase == 2) ? h - 1 : nf - 1;
+#pragma unroll
+        for (int k = 0; k < kAdjPer; ++k) {
+            const int w = tid + k * 256;
+            if (w < cnt) {
+                B(i0)[w] = v0[k];
+                if (phase != 1) B(i0 + 1)[w] = v1[k];
+            }
+        }
+        __syncthreads();
+        if (r + 1 < r1) load(r + 1);
 
         if (phase == 1) {
-            const long long s0 = ck * sh.C1;
-            const int C = (int)min((long long)sh.C1, ES - s0);
-            const int cnt = C * (int)EF;
-            for (int w = tid; w < cnt; w += nthr) B(0)[w] = dy[s0 * EF + w];
-            __syncthreads();
-            ext[0] = C;
-            for (int t = 0; t < nf; ++t) ext[1 + t] = n;
+            // the slab chunk is C x n^nf; fast mode t sees outer C n^t, inner n^(nf-1-t)
             int cur = 0;
             for (int t = 0; t < nf; ++t) {
-                contract_mode_lds<T, Acc, true, SQ>(B(cur), B(1 - cur), ext, 1 + nf, 1 + t, n,
-                                                 fac(h + t), n);
+                contract_mode_lds<T, Acc, true, SQ>(B(cur), B(1 - cur), C * ipow(n, t), n,
+                                                    ipow(n, nf - 1 - t), n, fac(h + t), n);
                 __syncthreads();
                 cur = 1 - cur;
             }
+            T *V = W + (long long)b * 2 * E;
             for (int w = tid; w < cnt; w += nthr) V[s0 * EF + w] = B(cur)[w];
-            __syncthreads();
-            continue;
-        }
-
-        if (phase == 2) {
-            // buffers: P_i in B(i) (B(h-1) holds X), G ping-pong in B(h), B(h+1)
-            const T *Vr = (nf > 0) ? V : dy;
-            const long long c0 = ck * sh.C2;
-            const int C = (int)min((long long)sh.C2, EF - c0);
-            const int cnt = (int)ES * C;
-            for (int w = tid; w < cnt; w += nthr) {
-                const int is = w / C, c = w % C;
-                B(h - 1)[w] = x[is * EF + c0 + c];
-                B(h)[w] = Vr[is * EF + c0 + c];
-            }
-            __syncthreads();
-            for (int k = 0; k < h; ++k) ext[k] = n;
-            ext[h] = C;
+        } else if (phase == 2) {
+            // P_i in B(i) (B(h-1) holds X), G ping-pong in B(h), B(h+1); the
+            // column chunk is n^h x C, slow mode i sees outer n^i, inner n^(h-1-i) C
             for (int i = h - 2; i >= 0; --i) {
-                contract_mode_lds<T, Acc, false, SQ>(B(i + 1), B(i), ext, h + 1, i + 1, n,
-                                                  fac(i + 1), n);
+                contract_mode_lds<T, Acc, false, SQ>(B(i + 1), B(i), ipow(n, i + 1), n,
+                                                     ipow(n, h - 2 - i) * C, n, fac(i + 1), n);
                 __syncthreads();
             }
             if (nf > 0) {
                 // U chunk = A_0 x_0 P_0, through B(h+1) (the G chain starts in B(h))
-                contract_mode_lds<T, Acc, false, SQ>(B(0), B(h + 1), ext, h + 1, 0, n, fac(0), n);
+                contract_mode_lds<T, Acc, false, SQ>(B(0), B(h + 1), 1, n, ipow(n, h - 1) * C, n,
+                                                     fac(0), n);
                 __syncthreads();
+                T *U = W + (long long)b * 2 * E + E;
                 for (int w = tid; w < cnt; w += nthr) {
                     const int is = w / C, c = w % C;
-                    U[is * EF + c0 + c] = B(h + 1)[w];
+                    U[(long long)is * EF + s0 + c] = B(h + 1)[w];
                 }
                 __syncthreads();
             }
             int g = h;
             for (int i = 0; i < h; ++i) {
-                adj_gram<T>(B(g), B(i), ext, h + 1, i, dAacc + i * n * n);
+                const int outer = ipow(n, i), inner = ipow(n, h - 1 - i) * C;
+                adj_gram<T>(B(g), B(i), outer, n, inner, dAacc + i * n * n);
                 const int gn = (g == h) ? h + 1 : h;
-                contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), ext, h + 1, i, n, fac(i), n);
+                contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n, fac(i), n);
                 __syncthreads();
                 g = gn;
             }
             T *dx = dX + b * E;
             for (int w = tid; w < cnt; w += nthr) {
                 const int is = w / C, c = w % C;
-                dx[is * EF + c0 + c] = B(g)[w];
+                dx[(long long)is * EF + s0 + c] = B(g)[w];
             }
         } else {
-            // phase 3 -- buffers: P_t in B(t) (B(nf-1) holds U), G ping-pong in
-            // B(nf), B(nf+1)
-            const long long s0 = ck * sh.C1;
-            const int C = (int)min((long long)sh.C1, ES - s0);
-            const int cnt = C * (int)EF;
-            for (int w = tid; w < cnt; w += nthr) {
-                B(nf - 1)[w] = U[s0 * EF + w];
-                B(nf)[w] = dy[s0 * EF + w];
-            }
-            __syncthreads();
-            ext[0] = C;
-            for (int t = 0; t < nf; ++t) ext[1 + t] = n;
+            // P_t in B(t) (B(nf-1) holds U), G ping-pong in B(nf), B(nf+1); the
+            // slab chunk is C x n^nf, fast mode t sees outer C n^t, inner n^(nf-1-t)
             for (int t = nf - 2; t >= 0; --t) {
-                contract_mode_lds<T, Acc, false, SQ>(B(t + 1), B(t), ext, 1 + nf, 2 + t, n,
-                                                  fac(h + t + 1), n);
+                contract_mode_lds<T, Acc, false, SQ>(B(t + 1), B(t), C * ipow(n, t + 1), n,
+                                                     ipow(n, nf - 2 - t), n, fac(h + t + 1), n);
                 __syncthreads();
             }
             int g = nf;
             for (int t = 0; t < nf; ++t) {
-                adj_gram<T>(B(g), B(t), ext, 1 + nf, 1 + t, dAacc + (h + t) * n * n);
+                const int outer = C * ipow(n, t), inner = ipow(n, nf - 1 - t);
+                adj_gram<T>(B(g), B(t), outer, n, inner, dAacc + (h + t) * n * n);
                 if (t + 1 < nf) {
                     const int gn = (g == nf) ? nf + 1 : nf;
-                    contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), ext, 1 + nf, 1 + t, n,
-                                                     fac(h + t), n);
+                    contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n,
+                                                        fac(h + t), n);
                     g = gn;
                 }
                 __syncthreads();
             }
         }
         __syncthreads();
-        // this item's share of dA: modes [0, h) in phase 2, [h, d) in phase 3
-        const int k0 = (phase == 2) ? 0 : h, k1 = (phase == 2) ? h : d;
-        const auto dAb = to_global(dA + (long long)b * d * n * n);
-        for (int w = k0 * n * n + tid; w < k1 * n * n; w += nthr) atomic_add(dAb + w, T(dAacc[w]));
-        __syncthreads();
     }
+    if (cur_b >= 0 && phase != 1) flush(cur_b);
 }
 
 template <typename T>
@@ -231,7 +284,7 @@ static bool plan_adjoint(AdjShape &sh, long long budget)
         const int nf = d - h;
         const int nb2 = h + 2, nb3 = nf + 2;
         const int nbuf = std::max(nb2, nf > 0 ? std::max(2, nb3) : 0);
-        const long long per = budget / nbuf;  // elements per buffer
+        const long long per = std::min<long long>(budget / nbuf, kAdjPer * 256);
         if (per < std::max(s.ES, nf > 0 ? s.EF : 1)) continue;
         s.C1 = nf > 0 ? (int)std::min<long long>(s.ES, per / s.EF) : 1;
         s.C2 = (int)std::min<long long>(s.EF, per / s.ES);
@@ -297,7 +350,7 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
         if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
     }
     const long long resident = persistent_blocks(kern, lds);
-    const long long cap = resident > 0 ? 8 * resident : 65536;
+    const long long cap = resident > 0 ? 2 * resident : 4096;
     // dA is accumulated atomically by the work items: clear it first (a fill
     // kernel, so the sequence stays valid under stream capture)
     const long long nA = (long long)batch * d * n * n;

@@ -78,14 +78,6 @@ __device__ __forceinline__ T *uniform_ptr(T *p)
 {
     return reinterpret_cast<T *>(uniform_u64(reinterpret_cast<unsigned long long>(p)));
 }
-__device__ __forceinline__ double uniform_val(double v)
-{
-    return __builtin_bit_cast(double, uniform_u64(__builtin_bit_cast(unsigned long long, v)));
-}
-__device__ __forceinline__ float uniform_val(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, v)));
-}
 
 // Hardware FP atomic add (no-return global_atomic_add_f64 / _f32 on gfx950).
 // Y must be ordinary device memory (hipMalloc / torch), where these are exact
@@ -186,21 +178,17 @@ __device__ __forceinline__ void y_accumulate(P y, long long idx, Acc v)
     }
 }
 
-// dst = op(A) x_q src for mode q of an LDS tensor with extents ext[0..R):
-// ext[q] = n on input, m on output; A is m x n (n x m stored when TRANS) in
-// LDS with leading dimension ldA.  One thread per fiber: n loads into
-// registers (n <= NMAX), m outputs; consecutive threads take consecutive
-// fast indices.  Larger n falls back to one thread per output element.
+// dst = op(A) x_q src for one mode of an LDS tensor viewed as
+// outer x n x inner (input) and outer x m x inner (output), column-major with
+// the inner index fastest; A is m x n (n x m stored when TRANS) in LDS with
+// leading dimension ldA.  One thread per fiber: n loads into registers
+// (n <= NMAX), m outputs; consecutive threads take consecutive inner indices.
+// Larger n falls back to one thread per output element.
 template <typename T, typename Acc, bool TRANS, int NMAX>
 __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__restrict__ dst,
-                                                const int *ext, int R, int q, int m,
+                                                int outer, int n, int inner, int m,
                                                 const T *__restrict__ Al, int ldA)
 {
-    int inner = 1;
-    for (int r = q + 1; r < R; ++r) inner *= ext[r];
-    int outer = 1;
-    for (int r = 0; r < q; ++r) outer *= ext[r];
-    const int n = ext[q];
     if (n <= NMAX) {
         const int fibers = outer * inner;
         for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
@@ -242,27 +230,21 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
 // N*N FMAs each.  SQ == 0 selects contract_fibers (runtime extents).
 template <typename T, typename Acc, bool TRANS, int SQ>
 __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *__restrict__ dst,
-                                                  const int *ext, int R, int q, int m,
+                                                  int outer, int n, int inner, int m,
                                                   const T *__restrict__ Al, int ldA)
 {
     if constexpr (SQ == 0) {
-        contract_fibers<T, Acc, TRANS, 8>(src, dst, ext, R, q, m, Al, ldA);
+        contract_fibers<T, Acc, TRANS, 8>(src, dst, outer, n, inner, m, Al, ldA);
     } else {
         constexpr int N = SQ;
-        int inner = 1;
-        for (int r = q + 1; r < R; ++r) inner *= ext[r];
-        int outer = 1;
-        for (int r = 0; r < q; ++r) outer *= ext[r];
-        // the factor is the same for the whole workgroup: up to N = 6 it is kept
-        // in SGPRs (v_fma_f64 takes one scalar operand), above in VGPRs
+        // the factor is the same for the whole workgroup; it is held in VGPRs
+        // (a scalar copy of N*N doubles overflows the SGPR file above N = 3)
         Acc a[N][N];
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll
-            for (int j = 0; j < N; ++j) {
-                const Acc v = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
-                a[i][j] = (N <= 6) ? uniform_val(v) : v;
-            }
+            for (int j = 0; j < N; ++j)
+                a[i][j] = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
         const int fibers = outer * inner;
         for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
             const int ii = f % inner, o = f / inner;

@@ -74,6 +74,7 @@ constexpr int kStagePer = 8;
 template <typename T>
 struct RegStage {
     T v[kStagePer];
+    T y[kStagePer];  // pass 2 without atomics: the Y chunk to update
 };
 
 template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS, int SQ>
@@ -125,6 +126,18 @@ struct GenericPasses {
                 }
             }
         }
+        if constexpr (!SHARED) {
+            // the item's Y chunk is read now, so its latency overlaps too
+            if (phase == 2) {
+                const auto y = to_global(ent.y(b));
+                const int nout = C * (int)sh.nlo_out;
+#pragma unroll
+                for (int k = 0; k < kStagePer; ++k) {
+                    const int w = threadIdx.x + k * 256;
+                    if (w < nout) st.y[k] = y[c0 * sh.nlo_out + w];
+                }
+            }
+        }
     }
     __device__ __forceinline__ void to_lds(const RegStage<T> &st, int cnt) const
     {
@@ -143,19 +156,23 @@ struct GenericPasses {
         }
     }
     // contract the item's modes in LDS (input in buf0) and store the result
-    __device__ __forceinline__ void compute_store(int b, long long c0, int C) const
+    __device__ __forceinline__ void compute_store(int b, long long c0, int C,
+                                                  const RegStage<T> &ycur) const
     {
         const int tid = threadIdx.x, nthr = blockDim.x, h = sh.h, d = sh.d;
-        int ext[kMaxDim + 1];
         T *src = buf0, *dst = buf1;
         if (phase == 1) {
-            for (int k = 0; k < h; ++k) ext[k] = sh.n[k];
-            ext[h] = C;
+            // the chunk is n_0 x .. x n_{h-1} x C; mode q sees outer = m_0..m_{q-1}
+            // (already contracted) and inner = n_{q+1}..n_{h-1} x C
+            int outer = 1;
+            int inner = C;
+            for (int k = 1; k < h; ++k) inner *= sh.n[k];
             for (int q = 0; q < h; ++q) {
-                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, ext, h + 1, q, sh.m[q],
+                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, outer, sh.n[q], inner, sh.m[q],
                                                      facs + sh.fac_off[q],
                                                      TRANS ? sh.n[q] : sh.m[q]);
-                ext[q] = sh.m[q];
+                outer *= sh.m[q];
+                if (q + 1 < h) inner /= sh.n[q + 1];
                 __syncthreads();
                 T *t = src; src = dst; dst = t;
             }
@@ -166,23 +183,32 @@ struct GenericPasses {
                 S[(long long)hi * sh.nlo_in + c0 + c] = src[w];
             }
         } else {
-            ext[0] = C;
-            for (int k = h; k < d; ++k) ext[1 + k - h] = sh.n[k];
-            const int R = 1 + d - h;
+            // the chunk is C x n_h x .. x n_{d-1}; mode k sees outer = C x m_h..m_{k-1}
+            // and inner = n_{k+1}..n_{d-1}
+            int outer = C;
+            int inner = 1;
+            for (int k = h + 1; k < d; ++k) inner *= sh.n[k];
             for (int k = h; k < d; ++k) {
-                const int q = 1 + k - h;
-                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, ext, R, q, sh.m[k],
+                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, outer, sh.n[k], inner, sh.m[k],
                                                      facs + sh.fac_off[k],
                                                      TRANS ? sh.n[k] : sh.m[k]);
-                ext[q] = sh.m[k];
+                outer *= sh.m[k];
+                if (k + 1 < d) inner /= sh.n[k + 1];
                 __syncthreads();
                 T *t = src; src = dst; dst = t;
             }
             const auto y = to_global(ent.y(b));
             const int nout = C * (int)sh.nlo_out;
             const long long ybase = c0 * sh.nlo_out;
-            for (int w = tid; w < nout; w += nthr)
-                y_accumulate<SHARED, T, Acc>(y, ybase + w, Acc(src[w]));
+            if constexpr (SHARED) {
+                for (int w = tid; w < nout; w += nthr) atomic_add(y + (ybase + w), src[w]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kStagePer; ++k) {
+                    const int w = tid + k * 256;
+                    if (w < nout) y[ybase + w] = T(Acc(ycur.y[k]) + Acc(src[w]));
+                }
+            }
         }
     }
 };
@@ -221,9 +247,14 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
             cur_b = b;
         }
         P.to_lds(st, P.count(C));
+        RegStage<T> ycur;
+        if constexpr (!SHARED) {
+#pragma unroll
+            for (int k = 0; k < kStagePer; ++k) ycur.y[k] = st.y[k];
+        }
         __syncthreads();
         if (r + 1 < r1) P.load(st, r + 1);
-        P.compute_store(b, c0, C);
+        P.compute_store(b, c0, C, ycur);
         __syncthreads();
     }
 }
