@@ -147,6 +147,82 @@ kron_status kron_xbatched_plan_execute(kron_xplan plan, kron_stream_t stream);
 kron_status kron_xbatched_plan_destroy(kron_xplan plan);
 
 /*
+ * Mixed-shape plan: many square (d, n) classes applied in one persistent
+ * launch (no counterpart in the reference, where each class is a separate
+ * kronmult{d}_vbatched launch, kronmult_vbatched.hpp:19-248).  Every class
+ * has kronmult_xbatched / vbatched semantics for square factors:
+ *   Aarray[k + b*d] -> n x n factor k of entry b, leading dimension lda
+ *   pX[b] -> n^d input, pY[b] -> n^d output, Y += kron(A) X
+ * with element-wise atomic adds wherever entries (of any classes) may share
+ * a Y vector.  plan_create copies the pY arrays to the host once (it
+ * synchronises `stream`), groups entries by Y target, checks that distinct
+ * targets do not overlap, picks each class's tile / thread mapping and sorts
+ * the work items by cost, largest first.  Classes outside the compiled set
+ * (d = 6 with n >= 7, n > 8) run as extra launches on the same stream.
+ *
+ * tune_cache: NULL, or a text file of tuned mappings keyed by GPU
+ * architecture, element type, d and n (created if absent).
+ * flags: KRON_MIXED_TUNE times every mapping of each class not in the
+ *   cache on synthetic data (the caller's arrays are not touched) and adds
+ *   the result to the cache; without it uncached classes use the default
+ *   mapping and a byte-count cost model.
+ *   KRON_MIXED_ATOMIC forces atomic Y updates everywhere;
+ *   KRON_MIXED_FORCE_V0 / _V1 force mapping 0 / 1 where it exists (tests).
+ * The contents of A, X and Y may change between executes; the pointer
+ * arrays must not.  plan_capture records `applies` executes in a hipGraph
+ * (on a plan-owned stream); plan_replay launches it on `stream`.
+ * plan_destroy synchronises the device and frees the plan's memory.
+ */
+typedef struct kron_mixed_class_f64 {
+    int d;                          /* factors per entry, 1..6 */
+    int n;                          /* factor size (n x n) */
+    int lda;                        /* leading dimension, >= n (0 means n) */
+    int batchCount;                 /* entries in the class (>= 0) */
+    const double *const *Aarray;    /* device: d*batchCount factor pointers */
+    const double *const *pX;        /* device: batchCount input pointers */
+    double *const *pY;              /* device: batchCount output pointers */
+} kron_mixed_class_f64;
+typedef struct kron_mixed_class_f32 {
+    int d, n, lda, batchCount;
+    const float *const *Aarray;
+    const float *const *pX;
+    float *const *pY;
+} kron_mixed_class_f32;
+
+#define KRON_MIXED_TUNE 0x1
+#define KRON_MIXED_ATOMIC 0x2
+#define KRON_MIXED_FORCE_V0 0x100
+#define KRON_MIXED_FORCE_V1 0x200
+
+/* How a class of a mixed plan runs (kron_mixed_plan_class_info). */
+#define KRON_MIXED_KIND_LAUNCH 0   /* separate launch (outside the compiled set) */
+#define KRON_MIXED_KIND_ONCHIP 1   /* one entry per workgroup, entry in LDS */
+#define KRON_MIXED_KIND_PACKED 2   /* several entries per workgroup */
+#define KRON_MIXED_KIND_TWOPASS 3  /* slow / fast mode passes through scratch */
+
+typedef struct kron_mplan_s *kron_mplan;
+kron_status kron_mixed_plan_create_f64(kron_mplan *plan, int nclass,
+                                       const kron_mixed_class_f64 *classes,
+                                       const char *tune_cache, int flags,
+                                       kron_stream_t stream);
+kron_status kron_mixed_plan_create_f32(kron_mplan *plan, int nclass,
+                                       const kron_mixed_class_f32 *classes,
+                                       const char *tune_cache, int flags,
+                                       kron_stream_t stream);
+kron_status kron_mixed_plan_execute(kron_mplan plan, kron_stream_t stream);
+kron_status kron_mixed_plan_capture(kron_mplan plan, int applies);
+kron_status kron_mixed_plan_replay(kron_mplan plan, kron_stream_t stream);
+/* kind: KRON_MIXED_KIND_*; variant: mapping index; atomic: 1 if Y updates
+ * of the class are atomic; cost: estimated cost of one entry (microseconds
+ * of the whole GPU when tuned, else bytes); any output pointer may be NULL. */
+kron_status kron_mixed_plan_class_info(kron_mplan plan, int cls, int *kind, int *variant,
+                                       int *atomic, double *cost);
+/* Work items, persistent workgroups and dynamic LDS bytes of the launch. */
+kron_status kron_mixed_plan_launch_info(kron_mplan plan, int *items, int *grid,
+                                        int *lds_bytes);
+kron_status kron_mixed_plan_destroy(kron_mplan plan);
+
+/*
  * Batched small GEMM with per-entry pointers and leading dimensions (device
  * arrays), uniform mm, nn, kk.  Replaces kgemm_nn_batched
  * (kgemm_nn_batched.cpp:9-42) and kgemm_nt_batched (kgemm_nt_batched.cpp:8-41).

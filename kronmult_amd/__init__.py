@@ -20,7 +20,7 @@ import functools
 import os
 
 __all__ = [
-    "KronError", "lib", "ptr_array", "XbatchedPlan",
+    "KronError", "lib", "ptr_array", "XbatchedPlan", "MixedPlan", "MixedClass",
     "kronmult_batched", "kronmult_batched_adjoint", "kronmult_xbatched", "kronmult_vbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
 ] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
@@ -78,6 +78,18 @@ def lib():
     L.kron_xbatched_plan_execute.restype = _c_int
     L.kron_xbatched_plan_destroy.argtypes = [_c_vp]
     L.kron_xbatched_plan_destroy.restype = _c_int
+    for sfx in ("f64", "f32"):
+        f = getattr(L, "kron_mixed_plan_create_" + sfx)
+        f.argtypes = [ctypes.POINTER(_c_vp), _c_int, _c_vp, ctypes.c_char_p, _c_int, _c_vp]
+        f.restype = _c_int
+    for name, args in (("execute", [_c_vp, _c_vp]), ("capture", [_c_vp, _c_int]),
+                       ("replay", [_c_vp, _c_vp]), ("destroy", [_c_vp]),
+                       ("class_info", [_c_vp, _c_int, _c_ip, _c_ip, _c_ip,
+                                       ctypes.POINTER(ctypes.c_double)]),
+                       ("launch_info", [_c_vp, _c_ip, _c_ip, _c_ip])):
+        f = getattr(L, "kron_mixed_plan_" + name)
+        f.argtypes = args
+        f.restype = _c_int
     L.kron_status_string.argtypes = [_c_int]
     L.kron_status_string.restype = ctypes.c_char_p
     L.kron_last_hip_error.restype = _c_int
@@ -205,6 +217,87 @@ class XbatchedPlan:
     def close(self):
         if self._h:
             lib().kron_xbatched_plan_destroy(self._h)
+            self._h = _c_vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+KRON_MIXED_TUNE, KRON_MIXED_ATOMIC = 0x1, 0x2
+KRON_MIXED_FORCE_V0, KRON_MIXED_FORCE_V1 = 0x100, 0x200
+MIXED_KINDS = {0: "launch", 1: "onchip", 2: "packed", 3: "twopass"}
+
+
+class _CMixedClass(ctypes.Structure):
+    # kron_mixed_class_f64 / _f32 (include/kronmult_amd.h)
+    _fields_ = [("d", _c_int), ("n", _c_int), ("lda", _c_int), ("batchCount", _c_int),
+                ("Aarray", _c_vp), ("pX", _c_vp), ("pY", _c_vp)]
+
+
+class MixedClass:
+    """One square (d, n) class of a mixed plan: xbatched-style pointer arrays
+    (Aarray[k + b*d], pX[b], pY[b]) on the device."""
+
+    def __init__(self, d, n, Aarray, Xarray, Yarray, batchCount, lda=0):
+        self.d, self.n, self.lda, self.batchCount = d, n, lda, batchCount
+        self.Aarray, self.Xarray, self.Yarray = Aarray, Xarray, Yarray
+
+
+class MixedPlan:
+    """kron_mixed_plan_*: many (d, n) classes applied in one persistent launch.
+
+    tune_cache: path of the autotuner's cache file (or None); tune=True times
+    the mappings of classes missing from it.  capture(applies) records that
+    many executes in a hipGraph; replay() launches it."""
+
+    def __init__(self, classes, dtype=None, tune_cache=None, tune=False, flags=0, stream=None):
+        import torch
+        sfx = _sfx(dtype if dtype is not None else torch.float64)
+        self._h = _c_vp()
+        self._keep = list(classes)
+        arr = (_CMixedClass * max(1, len(classes)))()
+        for i, c in enumerate(classes):
+            arr[i] = _CMixedClass(c.d, c.n, c.lda, c.batchCount, _addr(c.Aarray),
+                                  _addr(c.Xarray), _addr(c.Yarray))
+        fl = flags | (KRON_MIXED_TUNE if tune else 0)
+        path = tune_cache.encode() if tune_cache else None
+        f = getattr(lib(), "kron_mixed_plan_create_" + sfx)
+        _check("kron_mixed_plan_create",
+               f(ctypes.byref(self._h), len(classes), ctypes.cast(arr, _c_vp), path, fl,
+                 _stream(stream)))
+
+    def execute(self, stream=None):
+        _check("kron_mixed_plan_execute", lib().kron_mixed_plan_execute(self._h, _stream(stream)))
+
+    def capture(self, applies):
+        _check("kron_mixed_plan_capture", lib().kron_mixed_plan_capture(self._h, applies))
+
+    def replay(self, stream=None):
+        _check("kron_mixed_plan_replay", lib().kron_mixed_plan_replay(self._h, _stream(stream)))
+
+    def class_info(self, cls):
+        kind, var, atom = _c_int(), _c_int(), _c_int()
+        cost = ctypes.c_double()
+        _check("kron_mixed_plan_class_info",
+               lib().kron_mixed_plan_class_info(self._h, cls, ctypes.byref(kind),
+                                                ctypes.byref(var), ctypes.byref(atom),
+                                                ctypes.byref(cost)))
+        return {"kind": MIXED_KINDS[kind.value], "variant": var.value, "atomic": atom.value,
+                "cost": cost.value}
+
+    def launch_info(self):
+        items, grid, lds = _c_int(), _c_int(), _c_int()
+        _check("kron_mixed_plan_launch_info",
+               lib().kron_mixed_plan_launch_info(self._h, ctypes.byref(items), ctypes.byref(grid),
+                                                 ctypes.byref(lds)))
+        return {"items": items.value, "grid": grid.value, "lds_bytes": lds.value}
+
+    def close(self):
+        if self._h:
+            lib().kron_mixed_plan_destroy(self._h)
             self._h = _c_vp()
 
     def __del__(self):

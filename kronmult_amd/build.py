@@ -3,8 +3,8 @@
     python -m kronmult_amd.build          # or kronmult_amd.build.build()
 
 Every .hip translation unit is compiled in parallel to an object file under
-kronmult_amd/_build/, then linked with the host C ABI (capi.cpp) into
-kronmult_amd/libkronmult_amd.so.
+kronmult_amd/_build/, then linked with the host-only units (capi.cpp, the C
+ABI; mixed_plan.cpp, the mixed-shape plan) into kronmult_amd/libkronmult_amd.so.
 """
 import concurrent.futures as cf
 import os
@@ -37,6 +37,10 @@ def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+def host_sources():
+    return sorted(f for f in os.listdir(CSRC) if f.endswith(".cpp"))
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
@@ -46,10 +50,12 @@ def build(jobs=None, verbose=False):
         obj = os.path.join(BUILD, src[:-4] + ".o")
         objs.append(obj)
         cmds.append(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-    capi_obj = os.path.join(BUILD, "capi.o")
-    cmds.append(["hipcc", "-O2", "-std=c++17", "-fPIC", "-c",
-                 os.path.join(CSRC, "capi.cpp"), "-o", capi_obj])
-    objs.append(capi_obj)
+    # host-only translation units (C ABI, plans)
+    for src in host_sources():
+        obj = os.path.join(BUILD, src[:-4] + ".o")
+        cmds.append(["hipcc", "-O2", "-std=c++17", "-fPIC", "-c",
+                     os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():

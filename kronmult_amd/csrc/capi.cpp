@@ -7,6 +7,7 @@
 #include <utility>
 
 #include "kron_internal.hpp"
+#include "kron_mixed.hpp"
 #include "../../include/kronmult.hpp"
 
 namespace kron {
@@ -409,6 +410,70 @@ kron_status kron_xbatched_plan_destroy(kron_xplan plan)
         (void)hipFree(plan->ws);
         (void)hipSetDevice(cur);
     }
+    delete plan;
+    return KRON_OK;
+}
+
+struct kron_mplan_s {
+    MixedPlanCore *core = nullptr;
+};
+
+kron_status kron_mixed_plan_create_f64(kron_mplan *plan, int nclass,
+                                       const kron_mixed_class_f64 *classes,
+                                       const char *tune_cache, int flags, kron_stream_t stream)
+{
+    if (!plan) return KRON_EINVAL;
+    *plan = nullptr;
+    MixedPlanCore *core = nullptr;
+    kron_status st = mixed_create_f64(core, nclass, classes, tune_cache, flags,
+                                      static_cast<hipStream_t>(stream));
+    if (st != KRON_OK) return st;
+    *plan = new kron_mplan_s{core};
+    return KRON_OK;
+}
+kron_status kron_mixed_plan_create_f32(kron_mplan *plan, int nclass,
+                                       const kron_mixed_class_f32 *classes,
+                                       const char *tune_cache, int flags, kron_stream_t stream)
+{
+    if (!plan) return KRON_EINVAL;
+    *plan = nullptr;
+    MixedPlanCore *core = nullptr;
+    kron_status st = mixed_create_f32(core, nclass, classes, tune_cache, flags,
+                                      static_cast<hipStream_t>(stream));
+    if (st != KRON_OK) return st;
+    *plan = new kron_mplan_s{core};
+    return KRON_OK;
+}
+kron_status kron_mixed_plan_execute(kron_mplan plan, kron_stream_t stream)
+{
+    if (!plan) return KRON_EINVAL;
+    return mixed_execute(plan->core, static_cast<hipStream_t>(stream));
+}
+kron_status kron_mixed_plan_capture(kron_mplan plan, int applies)
+{
+    if (!plan) return KRON_EINVAL;
+    return mixed_capture(plan->core, applies);
+}
+kron_status kron_mixed_plan_replay(kron_mplan plan, kron_stream_t stream)
+{
+    if (!plan) return KRON_EINVAL;
+    return mixed_replay(plan->core, static_cast<hipStream_t>(stream));
+}
+kron_status kron_mixed_plan_class_info(kron_mplan plan, int cls, int *kind, int *variant,
+                                       int *atomic, double *cost)
+{
+    if (!plan) return KRON_EINVAL;
+    return mixed_class_info(plan->core, cls, kind, variant, atomic, cost);
+}
+kron_status kron_mixed_plan_launch_info(kron_mplan plan, int *items, int *grid, int *lds_bytes)
+{
+    if (!plan) return KRON_EINVAL;
+    return mixed_launch_info(plan->core, items, grid, lds_bytes);
+}
+kron_status kron_mixed_plan_destroy(kron_mplan plan)
+{
+    if (!plan) return KRON_OK;
+    mixed_destroy(plan->core);
     delete plan;
     return KRON_OK;
 }

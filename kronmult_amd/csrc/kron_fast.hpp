@@ -34,26 +34,36 @@ struct FastGeom {
 };
 
 // a[i][j] = A(i,j) (or A(j,i) when TRANS), A column-major with leading dim lda.
-// With one entry per workgroup Ap is a constant-address-space pointer: the
-// factor is read with scalar loads into SGPRs (v_fma_f64 takes one scalar
-// operand), saving N*N*2 VGPRs.
-template <typename T, typename Acc, int N, bool TRANS, bool UNI, typename PtrT>
+// With one entry per workgroup Ap is a constant-address-space pointer: a
+// factor of at most SBYTES bytes is read with scalar loads into SGPRs
+// (v_fma_f64 takes one scalar operand), saving N*N*2 VGPRs.  A larger one
+// would overflow the SGPR file (spills through v_writelane / v_readlane), so
+// it is read with vector loads of the same uniform addresses instead.
+template <typename T, typename Acc, int N, bool TRANS, bool UNI, int SBYTES = 288,
+          typename PtrT>
 __device__ __forceinline__ void load_factor(Acc (&a)[N][N], PtrT Ap, int lda)
 {
+    if constexpr (UNI && (size_t)N * N * sizeof(Acc) > (size_t)SBYTES) {
+        const gptr<const T> Av = (gptr<const T>)(uintptr_t)Ap;
 #pragma unroll
-    for (int j = 0; j < N; ++j)
+        for (int j = 0; j < N; ++j)
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            a[i][j] = Acc(TRANS ? Ap[j + i * lda] : Ap[i + j * lda]);
-        }
+            for (int i = 0; i < N; ++i)
+                a[i][j] = Acc(TRANS ? Av[j + i * lda] : Av[i + j * lda]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                a[i][j] = Acc(TRANS ? Ap[j + i * lda] : Ap[i + j * lda]);
+    }
 }
 
-// One pass over the fibers of mode K owned by thread g of its entry.
-template <typename Acc, int D, int N, int K, int G, typename LoadF, typename StoreF>
-__device__ __forceinline__ void fiber_pass(int g, const Acc (&a)[N][N], LoadF &&ld, StoreF &&st)
+// One pass over F fibers of stride S (fiber f holds the N elements
+// base + j*S, base = f % S + (f / S) * S * N) owned by thread g of G.
+template <typename Acc, int N, int S, int F, int G, typename LoadF, typename StoreF>
+__device__ __forceinline__ void tile_pass(int g, const Acc (&a)[N][N], LoadF &&ld, StoreF &&st)
 {
-    constexpr int F = cpow(N, D - 1);
-    constexpr int S = cpow(N, D - K);
     constexpr int IT = (F + G - 1) / G;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -72,6 +82,14 @@ __device__ __forceinline__ void fiber_pass(int g, const Acc (&a)[N][N], LoadF &&
             }
         }
     }
+}
+
+// One pass over the fibers of mode K of an N^D entry owned by thread g of G.
+template <typename Acc, int D, int N, int K, int G, typename LoadF, typename StoreF>
+__device__ __forceinline__ void fiber_pass(int g, const Acc (&a)[N][N], LoadF &&ld, StoreF &&st)
+{
+    tile_pass<Acc, N, cpow(N, D - K), cpow(N, D - 1), G>(g, a, static_cast<LoadF &&>(ld),
+                                                           static_cast<StoreF &&>(st));
 }
 
 // LDS address of element i.  For N = 4 (and at least 64 elements) the two

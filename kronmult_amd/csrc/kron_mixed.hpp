@@ -1,0 +1,82 @@
+// kron_mixed.hpp -- types shared by the mixed-(d, n) persistent kernel
+// (kron_mixed.hip) and its host-side plan (mixed_plan.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+#include "kron_internal.hpp"
+
+namespace kron {
+
+constexpr int kMixThreads = 256;       // threads per workgroup
+constexpr int kMixLdsBytes = 65536;    // LDS of the largest path (2 workgroups per CU)
+constexpr int kMixVariants = 2;        // tile / thread mapping variants per (d, n)
+
+// One shape class as the kernel reads it (device table).
+template <typename T>
+struct MixedClass {
+    const T *const *pA;  // Aarray[k + b*d]
+    const T *const *pX;  // pX[b]
+    T *const *pY;        // pY[b]
+    int d, n, lda;
+    int path;            // compiled path code (mixed_path_code)
+    int atomic_y;        // 1: Y updated with atomics
+    int pad;
+};
+
+// A work item: entries order[first .. first+count) of class cls.
+struct MixedItem {
+    int cls;
+    int first;
+    int count;
+    int pad;
+};
+
+// What a compiled (d, n, variant) path needs.
+struct MixPathInfo {
+    bool valid = false;
+    bool big = false;       // two passes through per-workgroup scratch
+    bool packed = false;    // several entries per workgroup, atomic Y
+    int per_item = 1;       // entries per item when packed
+    size_t lds = 0;         // dynamic LDS bytes
+    long long scratch = 0;  // scratch elements per workgroup
+    int split = 0;          // slow modes of pass 1 (big paths)
+    int threads = 0;        // threads per entry
+};
+
+struct MixedLaunch {
+    const void *classes = nullptr;  // MixedClass<T>[]
+    const MixedItem *items = nullptr;
+    const int *order = nullptr;
+    int nitems = 0;
+    unsigned *queue = nullptr;      // two counters, zero between launches
+    void *scratch = nullptr;
+    long long scratch_stride = 0;   // elements per workgroup
+    int grid = 0;
+    size_t lds = 0;
+};
+
+// The plan (mixed_plan.cpp), behind the C ABI's kron_mplan.
+struct MixedPlanCore;
+kron_status mixed_create_f64(MixedPlanCore *&out, int nclass, const kron_mixed_class_f64 *c,
+                             const char *cache, int flags, hipStream_t s);
+kron_status mixed_create_f32(MixedPlanCore *&out, int nclass, const kron_mixed_class_f32 *c,
+                             const char *cache, int flags, hipStream_t s);
+kron_status mixed_execute(MixedPlanCore *p, hipStream_t s);
+kron_status mixed_capture(MixedPlanCore *p, int applies);
+kron_status mixed_replay(MixedPlanCore *p, hipStream_t s);
+kron_status mixed_class_info(MixedPlanCore *p, int cls, int *kind, int *variant, int *atomic,
+                             double *cost);
+kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds);
+void mixed_destroy(MixedPlanCore *p);
+
+template <typename T>
+bool mixed_path_info(int d, int n, int variant, MixPathInfo &out);
+int mixed_path_code(int d, int n, int variant);
+template <typename T>
+const void *mixed_kernel();
+template <typename T>
+kron_status mixed_launch(const MixedLaunch &L, hipStream_t stream);
+
+}  // namespace kron

@@ -130,3 +130,24 @@ def norm_rel_err(got, want):
     den = np.max(np.abs(want)) if want.size else 0.0
     num = np.max(np.abs(got - want)) if want.size else 0.0
     return float(num / den) if den > 0 else float(num)
+
+
+def mixed_class_counts(total, seed, d_lo=2, d_hi=6, n_lo=3, n_hi=8, max_elems=65536):
+    """Mixed-shape workload: `total` entries with d uniform in [d_lo, d_hi] and
+    n uniform in [n_lo, n_hi], pairs with n^d > max_elems redrawn.  Returns
+    {(d, n): count} in (d, n) order."""
+    out = {}
+    got = 0
+    stream = 0
+    while got < total:
+        r = splitmix64(8 * total, sub_seed(seed, stream))
+        ds = d_lo + (r[0::2] % np.uint64(d_hi - d_lo + 1)).astype(np.int64)
+        ns = n_lo + (r[1::2] % np.uint64(n_hi - n_lo + 1)).astype(np.int64)
+        keep = ns.astype(np.float64) ** ds <= max_elems
+        for d, n in zip(ds[keep], ns[keep]):
+            if got == total:
+                break
+            out[(int(d), int(n))] = out.get((int(d), int(n)), 0) + 1
+            got += 1
+        stream += 1
+    return dict(sorted(out.items()))
