@@ -242,6 +242,99 @@ def bench_adjoint(args):
                       "adjoint_alg_gbs": adj_bytes / (ms_a * 1e-3) / 1e9}), flush=True)
 
 
+def bench_mixed(args):
+    """Mixed-shape workload: 8,192 entries with d uniform in 2..6 and n in
+    3..8 (n^d <= 65,536 enforced by redrawing: 28 classes), A and X uniform
+    [-1,1], one X per entry, Y targets from a pool of 1,024 per class, FP64.
+    The persistent autotuned plan (50 applies per hipGraph replay) against
+    the reference's way: one vbatched launch per class per apply."""
+    import tempfile
+
+    import torch
+
+    import helpers as H
+    import kronmult_amd as K
+
+    dev = "cuda:0"
+    counts = H.mixed_class_counts(8192, 2024)
+    stream = torch.cuda.current_stream()
+    keep, classes, per_class = [], [], []
+    alg_bytes = 0
+    for i, ((d, n), batch) in enumerate(counts.items()):
+        E = n ** d
+        A = torch.from_numpy(_splitmix_uniform(batch * d * n * n, 4000 + 3 * i)).to(dev)
+        X = torch.from_numpy(_splitmix_uniform(batch * E, 4001 + 3 * i)).to(dev)
+        Y = torch.zeros((1024, E), dtype=torch.float64, device=dev)
+        y_map = _splitmix_index(batch, 4002 + 3 * i, 1024)
+        pA = A.data_ptr() + np.arange(batch * d, dtype=np.int64) * (n * n * 8)
+        pX = X.data_ptr() + np.arange(batch, dtype=np.int64) * (E * 8)
+        pY = Y.data_ptr() + y_map * (E * 8)
+        pA, pX, pY = (torch.from_numpy(p).to(dev) for p in (pA, pX, pY))
+        # vbatched scratch: room for every entry's slot of 2*n^d elements
+        W = torch.empty(2 * E * batch, dtype=torch.float64, device=dev)
+        keep += [A, X, Y, W, pA, pX, pY]
+        classes.append(K.MixedClass(d, n, pA, pX, pY, batch))
+        per_class.append((d, n, pA, pX, pY, W, batch))
+        alg_bytes += bytes_per_entry(d, n) * batch
+    entries = sum(counts.values())
+
+    def per_class_apply():
+        for d, n, pA, pX, pY, W, batch in per_class:
+            K.kronmult_vbatched([n] * d, [n] * d, pA, pX, pY, W, W.numel() * 8, batch,
+                                dtype=torch.float64, stream=stream)
+
+    cache = os.path.join(tempfile.gettempdir(), "kronmult_amd_tune_%d.txt" % os.getpid())
+    t0 = time.perf_counter()
+    plan = K.MixedPlan(classes, tune_cache=cache, tune=True, stream=stream)
+    tune_s = time.perf_counter() - t0
+    if os.path.exists(cache):
+        os.remove(cache)
+    applies = 50
+    ms_eager = _time_steps(lambda: plan.execute(stream), args.steps, args.warmup, stream)
+    plan.capture(applies)
+    reps = max(1, args.steps // 10)
+    ms_graph = _time_steps(lambda: plan.replay(stream), reps, 1, stream) / applies
+    ms_pc = _time_steps(per_class_apply, args.steps, args.warmup, stream)
+    # the per-class launches captured in a graph as well (launch overhead removed)
+    g = torch.cuda.CUDAGraph()
+    per_class_apply()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        for _ in range(applies):
+            for d, n, pA, pX, pY, W, batch in per_class:
+                K.kronmult_vbatched([n] * d, [n] * d, pA, pX, pY, W, W.numel() * 8, batch,
+                                    dtype=torch.float64)
+    ms_pc_graph = _time_steps(g.replay, reps, 1, stream) / applies
+    kinds = {}
+    for i, (d, n) in enumerate(counts):
+        info = plan.class_info(i)
+        kinds["%d,%d" % (d, n)] = "%s/v%d%s" % (info["kind"], info["variant"],
+                                               "/atomic" if info["atomic"] else "")
+    li = plan.launch_info()
+    plan.close()
+    out = {
+        "workload": "mixed vbatched: 8192 entries, 28 (d,n) classes, d 2..6, n 3..8, "
+                    "Ypool 1024/class, 50 applies per hipGraph",
+        "dtype": "f64",
+        "classes": len(counts),
+        "entries": entries,
+        "alg_bytes_per_apply": alg_bytes,
+        "persistent_graph_ms_per_apply": ms_graph,
+        "persistent_eager_ms_per_apply": ms_eager,
+        "per_class_launch_ms_per_apply": ms_pc,
+        "per_class_graph_ms_per_apply": ms_pc_graph,
+        "speedup_vs_per_class_launches": ms_pc / ms_graph,
+        "speedup_vs_per_class_graph": ms_pc_graph / ms_graph,
+        "entries_per_s": entries / (ms_graph * 1e-3),
+        "alg_gbs": alg_bytes / (ms_graph * 1e-3) / 1e9,
+        "frac": alg_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "tune_s": tune_s,
+        "launch": li,
+        "mapping": kinds,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_copy(args):
     """Device-to-device copy of 2 GiB: the achievable HBM read+write rate."""
     import torch
@@ -274,6 +367,8 @@ def main():
         bench_batched(args)
     elif args.workload == "adjoint_d6n6":
         bench_adjoint(args)
+    elif args.workload == "mixed_sparse_grid":
+        bench_mixed(args)
     elif args.workload == "copy":
         bench_copy(args)
     else:

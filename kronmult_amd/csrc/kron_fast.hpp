@@ -84,6 +84,117 @@ __device__ __forceinline__ void tile_pass(int g, const Acc (&a)[N][N], LoadF &&l
     }
 }
 
+// Fiber iterations whose loads are batched together: about 16 values (32
+// VGPRs in FP64) per batch.
+template <int N>
+constexpr int batch_iters()
+{
+    return 16 / N > 0 ? 16 / N : 1;
+}
+
+// tile_pass for passes that read HBM: the thread's fibers are taken in
+// batches of B, and the loads of batch b+1 are issued before batch b is
+// computed, so one batch's latency hides behind the previous batch's FMAs.
+template <typename T, typename Acc, int N, int S, int F, int G, typename LoadF, typename StoreF>
+__device__ __forceinline__ void tile_pass_in(int g, const Acc (&a)[N][N], LoadF &&ld, StoreF &&st)
+{
+    constexpr int IT = (F + G - 1) / G;
+    constexpr int B = batch_iters<N>();
+    constexpr int NB = (IT + B - 1) / B;
+    T in[2][B][N];
+    auto fetch = [&](int bi, T(&dst)[B][N]) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int f = g + (bi * B + q) * G;
+            if (bi * B + q < IT && ((F % G) == 0 || f < F)) {
+                const int base = (f % S) + (f / S) * (S * N);
+#pragma unroll
+                for (int j = 0; j < N; ++j) dst[q][j] = ld(base + j * S);
+            }
+        }
+    };
+    fetch(0, in[0]);
+#pragma unroll
+    for (int bi = 0; bi < NB; ++bi) {
+        if (bi + 1 < NB) fetch(bi + 1, in[(bi + 1) & 1]);
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int f = g + (bi * B + q) * G;
+            if (bi * B + q < IT && ((F % G) == 0 || f < F)) {
+                const int base = (f % S) + (f / S) * (S * N);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    Acc s = a[i][0] * Acc(in[bi & 1][q][0]);
+#pragma unroll
+                    for (int j = 1; j < N; ++j)
+                        s = __builtin_fma(a[i][j], Acc(in[bi & 1][q][j]), s);
+                    st(base + i * S, s);
+                }
+            }
+        }
+    }
+}
+
+// tile_pass whose outputs are added into Y (y[yi(e)] += value for tile
+// element e, skipped where !ok(e)).  Without atomics the Y values are read in
+// batches, batch b+1 issued before batch b is stored: the read-modify-write
+// costs about one round trip instead of one per fiber (the compiler cannot
+// move a Y load above an earlier Y store by itself; batches touch disjoint
+// fibers, so this order is safe).
+template <typename T, typename Acc, int N, int S, int F, int G, typename LoadF, typename YP,
+          typename IdxF, typename OkF>
+__device__ __forceinline__ void tile_pass_y(int g, const Acc (&a)[N][N], LoadF &&ld, YP y,
+                                            IdxF &&yi, OkF &&ok, bool atomic)
+{
+    constexpr int IT = (F + G - 1) / G;
+    constexpr int B = batch_iters<N>();
+    constexpr int NB = (IT + B - 1) / B;
+    if (atomic) {
+        tile_pass<Acc, N, S, F, G>(g, a, static_cast<LoadF &&>(ld), [&](int e, Acc v) {
+            if (ok(e)) atomic_add(y + yi(e), T(v));
+        });
+        return;
+    }
+    T yv[2][B][N];
+    auto fetch = [&](int bi, T(&dst)[B][N]) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int f = g + (bi * B + q) * G;
+            if (bi * B + q < IT && ((F % G) == 0 || f < F)) {
+                const int base = (f % S) + (f / S) * (S * N);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const int e = base + i * S;
+                    dst[q][i] = ok(e) ? y[yi(e)] : T(0);
+                }
+            }
+        }
+    };
+    fetch(0, yv[0]);
+#pragma unroll
+    for (int bi = 0; bi < NB; ++bi) {
+        if (bi + 1 < NB) fetch(bi + 1, yv[(bi + 1) & 1]);
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int f = g + (bi * B + q) * G;
+            if (bi * B + q < IT && ((F % G) == 0 || f < F)) {
+                const int base = (f % S) + (f / S) * (S * N);
+                Acc in[N];
+#pragma unroll
+                for (int j = 0; j < N; ++j) in[j] = Acc(ld(base + j * S));
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    Acc s = a[i][0] * in[0];
+#pragma unroll
+                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+                    const int e = base + i * S;
+                    if (ok(e)) y[yi(e)] = T(Acc(yv[bi & 1][q][i]) + s);
+                }
+            }
+        }
+    }
+}
+
 // One pass over the fibers of mode K of an N^D entry owned by thread g of G.
 template <typename Acc, int D, int N, int K, int G, typename LoadF, typename StoreF>
 __device__ __forceinline__ void fiber_pass(int g, const Acc (&a)[N][N], LoadF &&ld, StoreF &&st)

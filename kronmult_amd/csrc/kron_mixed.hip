@@ -11,7 +11,11 @@
 //   * items: a Y-target group of an on-chip class (its entries run one after
 //     another in one workgroup, so Y is updated with plain read-modify-write),
 //     P entries of a small class (packed into one workgroup, atomic Y), or
-//     a group of a large class (two passes through the workgroup's scratch);
+//     one tile of a large class's slow-mode pass (X -> scratch) or fast-mode
+//     pass (scratch -> Y, for a Y-target group);
+//   * with large classes present an apply is two launches of the kernel:
+//     the first holds every slow-mode tile, the second every fast-mode tile,
+//     and the whole-entry items are split between them to balance the two;
 //   * the items are sorted by estimated cost, largest first, and taken from
 //     an atomic counter (longest-processing-time-first list scheduling);
 //   * each class runs a compile-time (D, N) path with a tile and thread
@@ -105,23 +109,13 @@ struct MixPath {
 // live than a single-shape kernel does.
 constexpr int kMixSBytes = 128;
 
-// Y(i) += v for one element, atomic or plain.
-template <typename T, typename Acc>
-__device__ __forceinline__ void mix_y(gptr<T> y, int i, Acc v, bool atomic)
-{
-    if (atomic)
-        atomic_add(y + i, T(v));
-    else
-        y[i] = T(Acc(y[i]) + v);
-}
-
 // One entry per workgroup, the entry resident in LDS (the fast kernel's mode
 // order: 2 from HBM, D..3 in place, 1 into Y).  The entries of the item share
 // one Y target when the class updates Y without atomics; they run in order,
 // and each thread updates the same Y elements every time.
 template <typename T, typename Acc, int D, int N>
 __device__ __forceinline__ void mix_onchip(const MixedClass<T> &c, const int *order, int count,
-                                           T *smem)
+                                           T *smem, int tid)
 {
     const cptr<int> ord = to_const(order);
     const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
@@ -130,11 +124,12 @@ __device__ __forceinline__ void mix_onchip(const MixedClass<T> &c, const int *or
     auto L = [&](int i) -> T & { return smem[lds_index<D, N>(i)]; };
     for (int i = 0; i < count; ++i) {
         const int b = ord[i];
-        int g = threadIdx.x;
+        int g = tid;
         asm volatile("" : "+v"(g));
         load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 1), c.lda);
+        constexpr int F = cpow(N, D - 1);
         const auto x = ent.template x_g<true>(b);
-        fiber_pass<Acc, D, N, 2, kMixThreads>(
+        tile_pass_in<T, Acc, N, cpow(N, D - 2), F, kMixThreads>(
             g, a, [&](int e) { return x[e]; }, [&](int e, Acc v) { L(e) = T(v); });
         __syncthreads();
         for_modes_desc<D, 3>([&](auto kc) {
@@ -146,8 +141,9 @@ __device__ __forceinline__ void mix_onchip(const MixedClass<T> &c, const int *or
         });
         load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0), c.lda);
         const auto y = ent.template y_g<true>(b);
-        fiber_pass<Acc, D, N, 1, kMixThreads>(
-            g, a, [&](int e) { return L(e); }, [&](int e, Acc v) { mix_y<T>(y, e, v, atomic); });
+        tile_pass_y<T, Acc, N, F, F, kMixThreads>(
+            g, a, [&](int e) { return L(e); }, y, [](int e) { return e; },
+            [](int) { return true; }, atomic);
         __syncthreads();
     }
 }
@@ -156,15 +152,14 @@ __device__ __forceinline__ void mix_onchip(const MixedClass<T> &c, const int *or
 // updated atomically (entries of one item may share Y).
 template <typename T, typename Acc, int D, int N, int G>
 __device__ __forceinline__ void mix_packed(const MixedClass<T> &c, const int *order, int count,
-                                           T *smem)
+                                           T *smem, int tid)
 {
     constexpr int E = cpow(N, D);
     const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
-    const int slot = threadIdx.x / G;
+    const int slot = tid / G;
     const bool active = slot < count;
     const int b = active ? to_global(order)[slot] : 0;
-    int g = threadIdx.x % G;
-    asm volatile("" : "+v"(g));
+    const int g = tid % G;
     Acc a[N][N];
     if constexpr (D == 1) {
         if (active) {
@@ -204,118 +199,137 @@ __device__ __forceinline__ void mix_packed(const MixedClass<T> &c, const int *or
     }
 }
 
-// A large entry in two passes through the workgroup's own scratch vector W
-// (n^d elements): pass 1 contracts the slow modes 1..H of S x C1 column
-// tiles (X -> W), pass 2 the fast modes H+1..D of C2 x F slab tiles (W -> Y).
-// Tiles past the end of the entry compute on stale LDS and are not stored.
-template <typename T, typename Acc, int D, int N, int H, int C1, int C2>
-__device__ __forceinline__ void mix_twopass(const MixedClass<T> &c, const int *order, int count,
-                                            T *smem, T *wscr)
+// A large entry in two passes through its own scratch vector W (n^d
+// elements; entry b of the class at scratch + wofs + b * n^d).  Pass 1
+// contracts the slow modes 1..H of one S x C1 column tile (X -> W), pass 2
+// the fast modes H+1..D of one C2 x F slab tile (W -> Y); each tile is a work
+// item, and the two passes are separate launches (the launch boundary makes
+// W visible across the chip).  Tile elements past the end of the entry
+// compute on stale LDS and are not stored.
+template <typename T, typename Acc, int D, int N, int H, int C1>
+__device__ __forceinline__ void mix_pass1(const MixedClass<T> &c, int b, int chunk, T *smem,
+                                          T *scratch, int tid)
+{
+    constexpr int S = cpow(N, H), F = cpow(N, D - H);
+    constexpr int T1 = S * C1;
+    const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
+    const auto W = to_global(scratch + c.wofs + (long long)b * (S * F));
+    const auto x = ent.template x_g<true>(b);
+    const int c0 = chunk * C1;
+    Acc a[N][N];
+    auto L = [&](int i) -> T & { return smem[i]; };
+    const int g = tid;
+    // tile element t = s * C1 + c is entry element s * F + c0 + c
+    auto gidx = [&](int t) { return (t / C1) * F + c0 + t % C1; };
+    auto valid = [&](int t) { return c0 + t % C1 < F; };
+    if constexpr (H == 1) {
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0),
+                                                        c.lda);
+        tile_pass_in<T, Acc, N, C1, T1 / N, kMixThreads>(
+            g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
+            [&](int t, Acc v) {
+                if (valid(t)) W[gidx(t)] = T(v);
+            });
+    } else {
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H - 1),
+                                                        c.lda);
+        tile_pass_in<T, Acc, N, C1, T1 / N, kMixThreads>(
+            g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
+            [&](int t, Acc v) { L(t) = T(v); });
+        __syncthreads();
+        for_modes_desc<H - 1, 2>([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            load_factor<T, Acc, N, false, true, kMixSBytes>(
+                a, ent.template factor_g<true>(b, K - 1), c.lda);
+            tile_pass<Acc, N, cpow(N, H - K) * C1, T1 / N, kMixThreads>(
+                g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
+            __syncthreads();
+        });
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0),
+                                                        c.lda);
+        tile_pass<Acc, N, cpow(N, H - 1) * C1, T1 / N, kMixThreads>(
+            g, a, [&](int t) { return L(t); },
+            [&](int t, Acc v) {
+                if (valid(t)) W[gidx(t)] = T(v);
+            });
+    }
+    __syncthreads();  // LDS is rewritten by the next item
+}
+
+// Pass 2 of slab tile `chunk` for the entries of one item, in order (they
+// share one Y target when the class updates Y without atomics, and this
+// workgroup owns that slab of it).
+template <typename T, typename Acc, int D, int N, int H, int C2>
+__device__ __forceinline__ void mix_pass2(const MixedClass<T> &c, const int *order, int count,
+                                          int chunk, T *smem, T *scratch, int tid)
 {
     const cptr<int> ord = to_const(order);
     constexpr int S = cpow(N, H), F = cpow(N, D - H);
-    constexpr int T1 = S * C1, T2 = C2 * F;
+    constexpr int T2 = C2 * F;
     const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
     const bool atomic = c.atomic_y != 0;
-    const auto W = to_global(wscr);
+    const int r0 = chunk * C2;
+    const int nv = (S - r0 < C2 ? S - r0 : C2) * F;  // valid tile elements
     Acc a[N][N];
     auto L = [&](int i) -> T & { return smem[i]; };
     for (int i = 0; i < count; ++i) {
         const int b = ord[i];
-        const auto x = ent.template x_g<true>(b);
-        const auto y = ent.template y_g<true>(b);
-        // pass 1: tile element t = s * C1 + c is entry element s * F + c0 + c
-        for (int c0 = 0; c0 < F; c0 += C1) {
-            int g = threadIdx.x;
-            asm volatile("" : "+v"(g));
-            auto gidx = [&](int t) { return (t / C1) * F + c0 + t % C1; };
-            auto valid = [&](int t) { return c0 + t % C1 < F; };
-            if constexpr (H == 1) {
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0), c.lda);
-                tile_pass<Acc, N, C1, T1 / N, kMixThreads>(
-                    g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
-                    [&](int t, Acc v) {
-                        if (valid(t)) W[gidx(t)] = T(v);
-                    });
-            } else {
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H - 1),
-                                                    c.lda);
-                tile_pass<Acc, N, C1, T1 / N, kMixThreads>(
-                    g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
-                    [&](int t, Acc v) { L(t) = T(v); });
+        int g = tid;
+        asm volatile("" : "+v"(g));
+        // tile element t = r * F + f is entry element r0 * F + t
+        const auto w = to_global(scratch + c.wofs + (long long)b * (S * F) + (long long)r0 * F);
+        const auto yt = ent.template y_g<true>(b) + (long long)r0 * F;
+        auto ok = [&](int t) { return t < nv; };
+        auto ident = [](int t) { return t; };
+        if constexpr (D - H == 1) {
+            load_factor<T, Acc, N, false, true, kMixSBytes>(
+                a, ent.template factor_g<true>(b, D - 1), c.lda);
+            tile_pass_y<T, Acc, N, 1, T2 / N, kMixThreads>(
+                g, a, [&](int t) { return t < nv ? w[t] : T(0); }, yt, ident, ok, atomic);
+        } else {
+            load_factor<T, Acc, N, false, true, kMixSBytes>(
+                a, ent.template factor_g<true>(b, D - 1), c.lda);
+            tile_pass_in<T, Acc, N, 1, T2 / N, kMixThreads>(
+                g, a, [&](int t) { return t < nv ? w[t] : T(0); },
+                [&](int t, Acc v) { L(t) = T(v); });
+            __syncthreads();
+            for_modes_desc<D - 1, H + 2>([&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                load_factor<T, Acc, N, false, true, kMixSBytes>(
+                    a, ent.template factor_g<true>(b, K - 1), c.lda);
+                tile_pass<Acc, N, cpow(N, D - K), T2 / N, kMixThreads>(
+                    g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
                 __syncthreads();
-                for_modes_desc<H - 1, 2>([&](auto kc) {
-                    constexpr int K = decltype(kc)::value;
-                    load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, K - 1),
-                                                        c.lda);
-                    tile_pass<Acc, N, cpow(N, H - K) * C1, T1 / N, kMixThreads>(
-                        g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
-                    __syncthreads();
-                });
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0), c.lda);
-                tile_pass<Acc, N, cpow(N, H - 1) * C1, T1 / N, kMixThreads>(
-                    g, a, [&](int t) { return L(t); },
-                    [&](int t, Acc v) {
-                        if (valid(t)) W[gidx(t)] = T(v);
-                    });
-            }
-            __syncthreads();  // LDS reuse; W visible to the whole workgroup
+            });
+            load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H),
+                                                            c.lda);
+            tile_pass_y<T, Acc, N, cpow(N, D - H - 1), T2 / N, kMixThreads>(
+                g, a, [&](int t) { return L(t); }, yt, ident, ok, atomic);
         }
-        // pass 2: tile element t = r * F + f is entry element r0 * F + t
-        for (int r0 = 0; r0 < S; r0 += C2) {
-            int g = threadIdx.x;
-            asm volatile("" : "+v"(g));
-            const int nv = (S - r0 < C2 ? S - r0 : C2) * F;  // valid tile elements
-            const auto w = W + (long long)r0 * F;
-            const auto yt = y + (long long)r0 * F;
-            if constexpr (D - H == 1) {
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, D - 1),
-                                                    c.lda);
-                tile_pass<Acc, N, 1, T2 / N, kMixThreads>(
-                    g, a, [&](int t) { return t < nv ? w[t] : T(0); },
-                    [&](int t, Acc v) {
-                        if (t < nv) mix_y<T>(yt, t, v, atomic);
-                    });
-            } else {
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, D - 1),
-                                                    c.lda);
-                tile_pass<Acc, N, 1, T2 / N, kMixThreads>(
-                    g, a, [&](int t) { return t < nv ? w[t] : T(0); },
-                    [&](int t, Acc v) { L(t) = T(v); });
-                __syncthreads();
-                for_modes_desc<D - 1, H + 2>([&](auto kc) {
-                    constexpr int K = decltype(kc)::value;
-                    load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, K - 1),
-                                                        c.lda);
-                    tile_pass<Acc, N, cpow(N, D - K), T2 / N, kMixThreads>(
-                        g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
-                    __syncthreads();
-                });
-                load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H), c.lda);
-                tile_pass<Acc, N, cpow(N, D - H - 1), T2 / N, kMixThreads>(
-                    g, a, [&](int t) { return L(t); },
-                    [&](int t, Acc v) {
-                        if (t < nv) mix_y<T>(yt, t, v, atomic);
-                    });
-            }
-            __syncthreads();  // LDS and W are rewritten next
-        }
+        __syncthreads();  // LDS is rewritten next
     }
 }
 
+// chunk: -1 for a whole-entry item, c >= 0 for pass-1 tile c, -2 - c for
+// pass-2 tile c of a large entry.
 template <typename T, typename Acc, int D, int N, int V>
 __device__ __forceinline__ void mix_run(const MixedClass<T> &c, const int *ord, int count,
-                                        T *smem, T *wscr)
+                                        int chunk, T *smem, T *scratch, int tid)
 {
     using Pth = MixPath<T, D, N, V>;
     if constexpr (!Pth::VALID) {
         return;
     } else if constexpr (Pth::BIG) {
-        mix_twopass<T, Acc, D, N, Pth::SP.h, Pth::SP.c1, Pth::SP.c2>(c, ord, count, smem, wscr);
+        if (chunk >= 0)
+            mix_pass1<T, Acc, D, N, Pth::SP.h, Pth::SP.c1>(c, to_const(ord)[0], chunk, smem,
+                                                           scratch, tid);
+        else
+            mix_pass2<T, Acc, D, N, Pth::SP.h, Pth::SP.c2>(c, ord, count, -2 - chunk, smem,
+                                                           scratch, tid);
     } else if constexpr (Pth::P == 1 && D >= 2) {
-        mix_onchip<T, Acc, D, N>(c, ord, count, smem);
+        mix_onchip<T, Acc, D, N>(c, ord, count, smem, tid);
     } else {
-        mix_packed<T, Acc, D, N, Pth::G>(c, ord, count, smem);
+        mix_packed<T, Acc, D, N, Pth::G>(c, ord, count, smem, tid);
     }
 }
 
@@ -331,23 +345,29 @@ __device__ __forceinline__ void mix_run(const MixedClass<T> &c, const int *ord, 
 constexpr int mix_code(int D, int N, int V) { return (D * 16 + N) * kMixVariants + V; }
 
 template <typename T>
-__global__ void __launch_bounds__(kMixThreads)
+__global__ void __launch_bounds__(kMixThreads, 2)
 kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const int *order,
-                  int nitems, unsigned *queue, T *scratch, long long scratch_stride)
+                  int nitems, unsigned *queue, T *scratch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
     __shared__ int s_next;
     T *smem = reinterpret_cast<T *>(kron_smem);
-    T *wscr = scratch + (long long)blockIdx.x * scratch_stride;
+    // the next item's index is requested one item ahead, so the atomic's
+    // round trip overlaps the current item
+    unsigned ahead = 0;
+    if (threadIdx.x == 0) ahead = atomicAdd(queue, 1u);
     for (;;) {
-        if (threadIdx.x == 0) s_next = (int)atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) {
+            s_next = (int)ahead;
+            ahead = atomicAdd(queue, 1u);
+        }
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_next);
         __syncthreads();  // s_next is rewritten by the next dequeue
         if (it >= nitems) break;
         // the item and its class, field by field through scalar loads
         const auto ip = to_const(items) + it;
-        const int ci = ip->cls, count = ip->count;
+        const int ci = ip->cls, count = ip->count, chunk = ip->chunk;
         const int *ord = order + ip->first;
         const auto cp = to_const(cls) + ci;
         MixedClass<T> c;
@@ -359,10 +379,16 @@ kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const int *o
         c.lda = cp->lda;
         c.path = cp->path;
         c.atomic_y = cp->atomic_y;
+        c.wofs = cp->wofs;
+        // lane-dependent index math of every path derives from this per-item
+        // opaque copy, so none of it is hoisted out of the loop (where it
+        // would stay live across all paths)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
         switch (c.path) {
 #define KRON_MIX_CASE_V(D, N, V)                                                              \
     case mix_code(D, N, V):                                                                   \
-        mix_run<T, T, D, N, V>(c, ord, count, smem, wscr);                                    \
+        mix_run<T, T, D, N, V>(c, ord, count, chunk, smem, scratch, tid);                     \
         break;
 #define KRON_MIX_CASE(D, N) KRON_MIX_CASE_V(D, N, 0) KRON_MIX_CASE_V(D, N, 1)
             KRON_MIX_FOR_EACH(KRON_MIX_CASE)
@@ -371,7 +397,7 @@ kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const int *o
         default: break;
         }
     }
-    // every workgroup has taken its last (empty) item; the last one out
+    // every workgroup has taken its last (empty) items; the last one out
     // clears the counters for the next launch on the stream
     if (threadIdx.x == 0) {
         __threadfence();
@@ -400,6 +426,8 @@ static bool path_info(int D, int N, int V, MixPathInfo &out)
         out.lds = Pth::VALID ? Pth::lds_bytes() : 0;                                          \
         out.scratch = Pth::VALID ? Pth::scratch_elems() : 0;                                  \
         out.split = Pth::BIG ? Pth::SP.h : 0;                                                 \
+        out.tiles1 = Pth::BIG ? (cpow(NN, DD - Pth::SP.h) + Pth::SP.c1 - 1) / Pth::SP.c1 : 0;  \
+        out.tiles2 = Pth::BIG ? (cpow(NN, Pth::SP.h) + Pth::SP.c2 - 1) / Pth::SP.c2 : 0;       \
         out.threads = Pth::G;                                                                 \
         return true;                                                                          \
     }
@@ -438,7 +466,7 @@ kron_status mixed_launch(const MixedLaunch &L, hipStream_t stream)
     }
     hipLaunchKernelGGL(kron_mixed_kernel<T>, dim3((unsigned)L.grid), dim3(kMixThreads), L.lds,
                        stream, static_cast<const MixedClass<T> *>(L.classes), L.items, L.order,
-                       L.nitems, L.queue, static_cast<T *>(L.scratch), L.scratch_stride);
+                       L.nitems, L.queue, static_cast<T *>(L.scratch));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_last_hip_error((int)e);

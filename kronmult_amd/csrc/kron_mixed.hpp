@@ -23,14 +23,16 @@ struct MixedClass {
     int path;            // compiled path code (mixed_path_code)
     int atomic_y;        // 1: Y updated with atomics
     int pad;
+    long long wofs;      // scratch offset of entry 0 (two-pass classes)
 };
 
-// A work item: entries order[first .. first+count) of class cls.
+// A work item: entries order[first .. first+count) of class cls (one entry
+// for a pass-1 tile).
 struct MixedItem {
     int cls;
     int first;
     int count;
-    int pad;
+    int chunk;  // -1: whole entries; c >= 0: pass-1 tile c; -2 - c: pass-2 tile c
 };
 
 // What a compiled (d, n, variant) path needs.
@@ -42,6 +44,8 @@ struct MixPathInfo {
     size_t lds = 0;         // dynamic LDS bytes
     long long scratch = 0;  // scratch elements per workgroup
     int split = 0;          // slow modes of pass 1 (big paths)
+    int tiles1 = 0;         // pass-1 tiles per entry (big paths)
+    int tiles2 = 0;         // pass-2 tiles per entry (big paths)
     int threads = 0;        // threads per entry
 };
 
@@ -51,8 +55,7 @@ struct MixedLaunch {
     const int *order = nullptr;
     int nitems = 0;
     unsigned *queue = nullptr;      // two counters, zero between launches
-    void *scratch = nullptr;
-    long long scratch_stride = 0;   // elements per workgroup
+    void *scratch = nullptr;        // per-entry scratch of two-pass classes
     int grid = 0;
     size_t lds = 0;
 };

@@ -126,7 +126,7 @@ struct MixedPlanCore {
         double cost = 0;
     };
     std::vector<Cls> cls;
-    MixedLaunch L;
+    MixedLaunch L[2];  // pass-1 tiles + part of the rest, pass-2 tiles + the rest
     std::vector<void *> allocs;
     // classes outside the compiled set
     std::vector<KronArgs<double>> launch64;
@@ -166,8 +166,9 @@ std::vector<KronArgs<float>> &launch_list<float>(MixedPlanCore &p) { return p.la
 template <typename T>
 static kron_status plan_execute_t(MixedPlanCore &p, hipStream_t s)
 {
-    kron_status st = mixed_launch<T>(p.L, s);
-    if (st != KRON_OK) return st;
+    kron_status st = KRON_OK;
+    for (const MixedLaunch &L : p.L)
+        if ((st = mixed_launch<T>(L, s)) != KRON_OK) return st;
     for (KronArgs<T> a : launch_list<T>(p)) {
         a.stream = s;
         st = launch_kron(a);
@@ -288,7 +289,6 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
     // 1. each class's path and mapping
     if (cache) load_cache(cache);
     size_t lds = 16;
-    long long scratch_stride = 0;
     for (int c = 0; c < nclass; ++c) {
         const ClassDesc &cd = classes[c];
         MixedPlanCore::Cls &C = p->cls[c];
@@ -326,7 +326,6 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         C.cost = (t.us > 0 && t.variant == v) ? t.us : model_cost_us<T>(cd.d, cd.n, info);
         C.atomic = (info.packed || (flags & KRON_MIXED_ATOMIC)) ? 1 : 0;
         lds = std::max(lds, info.lds);
-        scratch_stride = std::max(scratch_stride, info.scratch);
     }
 
     // 2. Y targets on the host: group each class's entries by target and prove
@@ -370,10 +369,17 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
             p->cls[ivs[j].cls].atomic = 1;
         }
 
-    // 3. work items
+    // 3. work items.  Launch 0 holds the pass-1 tiles of the two-pass
+    //    classes, launch 1 their pass-2 tiles; the whole-entry items go to
+    //    whichever launch is lighter (largest first), so both stay full.
     std::vector<MixedItem> items;
     std::vector<double> icost;
+    std::vector<int> ilaunch;
     std::vector<int> order;
+    long long wtotal = 0;  // scratch elements of the two-pass classes
+    std::vector<long long> wofs(nclass, 0);
+    double load[2] = {0, 0};
+    std::vector<int> whole;  // indices of whole-entry items
     for (int c = 0; c < nclass; ++c) {
         MixedPlanCore::Cls &C = p->cls[c];
         if (C.kind == KRON_MIXED_KIND_LAUNCH || C.desc.batch == 0) continue;
@@ -383,11 +389,19 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         order.insert(order.end(), ord.begin(), ord.end());
         MixPathInfo info;
         mixed_path_info<T>(C.desc.d, C.desc.n, C.variant, info);
+        auto add = [&](MixedItem it, double cost, int launch) {
+            items.push_back(it);
+            icost.push_back(cost);
+            ilaunch.push_back(launch);
+            if (launch < 0)
+                whole.push_back((int)items.size() - 1);
+            else
+                load[launch] += cost;
+        };
         if (C.kind == KRON_MIXED_KIND_PACKED) {
             for (int i = 0; i < C.desc.batch; i += info.per_item) {
                 const int cnt = std::min(info.per_item, C.desc.batch - i);
-                items.push_back(MixedItem{c, base + i, cnt, 0});
-                icost.push_back(C.cost * cnt);
+                add(MixedItem{c, base + i, cnt, -1}, C.cost * cnt, -1);
             }
             continue;
         }
@@ -402,19 +416,47 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         }
         if (longest > kMaxGroup) C.atomic = 1;
         const int cap = !C.atomic ? C.desc.batch : longest > kMaxGroup ? kMaxGroup : 1;
+        // a two-pass entry moves X in and W out in pass 1 (2 vectors), W and
+        // Y in and Y out in pass 2 (3 vectors): its cost splits 2 : 3
+        if (C.kind == KRON_MIXED_KIND_TWOPASS) {
+            wofs[c] = wtotal;
+            wtotal += (long long)C.desc.batch * ipow_ll(C.desc.n, C.desc.d);
+            for (int i = 0; i < C.desc.batch; ++i)
+                for (int t = 0; t < info.tiles1; ++t)
+                    add(MixedItem{c, base + i, 1, t}, 0.4 * C.cost / info.tiles1, 0);
+        }
         for (int i = 0; i < C.desc.batch;) {
             int j = i + 1;
             while (j < C.desc.batch && j - i < cap && keys[ord[j]] == keys[ord[i]]) ++j;
-            items.push_back(MixedItem{c, base + i, j - i, 0});
-            icost.push_back(C.cost * (j - i));
+            if (C.kind == KRON_MIXED_KIND_TWOPASS) {
+                for (int t = 0; t < info.tiles2; ++t)
+                    add(MixedItem{c, base + i, j - i, -2 - t},
+                        0.6 * C.cost * (j - i) / info.tiles2, 1);
+            } else {
+                add(MixedItem{c, base + i, j - i, -1}, C.cost * (j - i), -1);
+            }
             i = j;
         }
     }
-    std::vector<int> iord(items.size());
-    std::iota(iord.begin(), iord.end(), 0);
-    std::stable_sort(iord.begin(), iord.end(), [&](int a, int b) { return icost[a] > icost[b]; });
-    std::vector<MixedItem> sitems(items.size());
-    for (size_t i = 0; i < iord.size(); ++i) sitems[i] = items[iord[i]];
+    std::stable_sort(whole.begin(), whole.end(), [&](int a, int b) { return icost[a] > icost[b]; });
+    const bool two = wtotal > 0;
+    for (int w : whole) {
+        const int l = (two && load[1] < load[0]) ? 1 : 0;
+        ilaunch[w] = l;
+        load[l] += icost[w];
+    }
+    // each launch's items, largest first
+    std::vector<MixedItem> sitems;
+    int nA = 0;
+    for (int l = 0; l < 2; ++l) {
+        std::vector<int> iord;
+        for (size_t i = 0; i < items.size(); ++i)
+            if (ilaunch[i] == l) iord.push_back((int)i);
+        std::stable_sort(iord.begin(), iord.end(),
+                         [&](int a, int b) { return icost[a] > icost[b]; });
+        for (int i : iord) sitems.push_back(items[i]);
+        if (l == 0) nA = (int)sitems.size();
+    }
 
     // 4. device tables
     std::vector<MixedClass<T>> dcls(nclass);
@@ -431,8 +473,8 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
                                                   : mixed_path_code(C.desc.d, C.desc.n, C.variant);
         m.atomic_y = C.atomic;
         m.pad = 0;
+        m.wofs = wofs[c];
     }
-    MixedLaunch &L = p->L;
     hipError_t e = hipSuccess;
     void *dc = nullptr, *di = nullptr, *dord = nullptr, *dq = nullptr, *dscr = nullptr;
     auto put = [&](void **dst, const void *src, size_t bytes) {
@@ -442,33 +484,30 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
     put(&dc, dcls.data(), sizeof(MixedClass<T>) * dcls.size());
     put(&di, sitems.data(), sizeof(MixedItem) * sitems.size());
     put(&dord, order.data(), sizeof(int) * order.size());
-    if (e == hipSuccess && (e = hipMalloc(&dq, 2 * sizeof(unsigned))) == hipSuccess) {
+    if (e == hipSuccess && (e = hipMalloc(&dq, 4 * sizeof(unsigned))) == hipSuccess) {
         p->keep(dq);
-        e = hipMemset(dq, 0, 2 * sizeof(unsigned));
+        e = hipMemset(dq, 0, 4 * sizeof(unsigned));
     }
+    if (e == hipSuccess && wtotal > 0 && (e = hipMalloc(&dscr, wtotal * sizeof(T))) == hipSuccess)
+        p->keep(dscr);
     if (e != hipSuccess) {
         set_last_hip_error((int)e);
         return bail(KRON_EHIP);
     }
-    L.classes = dc;
-    L.items = static_cast<const MixedItem *>(di);
-    L.order = static_cast<const int *>(dord);
-    L.nitems = (int)sitems.size();
-    L.queue = static_cast<unsigned *>(dq);
-    L.lds = lds;
     const long long resident = persistent_blocks(mixed_kernel<T>(), lds);
-    long long grid = resident > 0 ? resident : 1024;
-    grid = std::max(1LL, std::min(grid, (long long)L.nitems));
-    L.grid = (int)grid;
-    L.scratch_stride = scratch_stride;
-    if (scratch_stride > 0 && L.nitems > 0) {
-        if ((e = hipMalloc(&dscr, (size_t)grid * scratch_stride * sizeof(T))) != hipSuccess) {
-            set_last_hip_error((int)e);
-            return bail(KRON_EHIP);
-        }
-        p->keep(dscr);
+    for (int l = 0; l < 2; ++l) {
+        MixedLaunch &L = p->L[l];
+        L.classes = dc;
+        L.items = static_cast<const MixedItem *>(di) + (l == 0 ? 0 : nA);
+        L.order = static_cast<const int *>(dord);
+        L.nitems = l == 0 ? nA : (int)sitems.size() - nA;
+        L.queue = static_cast<unsigned *>(dq) + 2 * l;
+        L.lds = lds;
+        long long grid = resident > 0 ? resident : 1024;
+        grid = std::max(1LL, std::min(grid, (long long)L.nitems));
+        L.grid = (int)grid;
+        L.scratch = dscr;
     }
-    L.scratch = dscr;
 
     // 5. classes outside the compiled set: their own launches, plan-owned scratch
     for (int c = 0; c < nclass; ++c) {
@@ -603,9 +642,9 @@ kron_status mixed_class_info(MixedPlanCore *p, int cls, int *kind, int *variant,
 kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds)
 {
     if (!p) return KRON_EINVAL;
-    if (items) *items = p->L.nitems;
-    if (grid) *grid = p->L.grid;
-    if (lds) *lds = (int)p->L.lds;
+    if (items) *items = p->L[0].nitems + p->L[1].nitems;
+    if (grid) *grid = std::max(p->L[0].grid, p->L[1].grid);
+    if (lds) *lds = (int)p->L[0].lds;
     return KRON_OK;
 }
 
