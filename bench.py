@@ -247,7 +247,10 @@ def bench_mixed(args):
     3..8 (n^d <= 65,536 enforced by redrawing: 28 classes), A and X uniform
     [-1,1], one X per entry, Y targets from a pool of 1,024 per class, FP64.
     The persistent autotuned plan (50 applies per hipGraph replay) against
-    the reference's way: one vbatched launch per class per apply."""
+    the reference's way, one vbatched launch per class per apply: with this
+    library's per-class kernels, and with the plain per-entry kernel (the
+    reference's structure: one workgroup per entry, one pass through W per
+    mode)."""
     import tempfile
 
     import torch
@@ -283,6 +286,11 @@ def bench_mixed(args):
             K.kronmult_vbatched([n] * d, [n] * d, pA, pX, pY, W, W.numel() * 8, batch,
                                 dtype=torch.float64, stream=stream)
 
+    def per_class_plain_apply():
+        for d, n, pA, pX, pY, W, batch in per_class:
+            K.kronmult_vbatched_plain([n] * d, [n] * d, pA, pX, pY, W, W.numel() * 8, batch,
+                                      dtype=torch.float64, stream=stream)
+
     cache = os.path.join(tempfile.gettempdir(), "kronmult_amd_tune_%d.txt" % os.getpid())
     t0 = time.perf_counter()
     plan = K.MixedPlan(classes, tune_cache=cache, tune=True, stream=stream)
@@ -295,6 +303,7 @@ def bench_mixed(args):
     reps = max(1, args.steps // 10)
     ms_graph = _time_steps(lambda: plan.replay(stream), reps, 1, stream) / applies
     ms_pc = _time_steps(per_class_apply, args.steps, args.warmup, stream)
+    ms_plain = _time_steps(per_class_plain_apply, max(1, args.steps // 4), 1, stream)
     # the per-class launches captured in a graph as well (launch overhead removed)
     g = torch.cuda.CUDAGraph()
     per_class_apply()
@@ -323,6 +332,8 @@ def bench_mixed(args):
         "persistent_eager_ms_per_apply": ms_eager,
         "per_class_launch_ms_per_apply": ms_pc,
         "per_class_graph_ms_per_apply": ms_pc_graph,
+        "plain_per_entry_ms_per_apply": ms_plain,
+        "speedup_vs_plain_per_entry": ms_plain / ms_graph,
         "speedup_vs_per_class_launches": ms_pc / ms_graph,
         "speedup_vs_per_class_graph": ms_pc_graph / ms_graph,
         "entries_per_s": entries / (ms_graph * 1e-3),

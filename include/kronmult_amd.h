@@ -122,6 +122,26 @@ kron_status kron_vbatched_f32(int d, const int *m, const int *n,
                               kron_stream_t stream);
 
 /*
+ * The same update with the reference's own per-entry structure on the GPU
+ * (kronmult_vbatched.hpp:164-247): one workgroup per entry, X copied into
+ * the entry's W slot, one pass through W per mode, the last mode added into
+ * Y atomically, sub-batches of Wcapacity / (2*sizeXW) entries.  Arguments
+ * and workspace rule as kron_vbatched_*.  A comparison baseline (the plain
+ * per-entry kernel the mixed-shape plan is measured against), not a fast
+ * path.
+ */
+kron_status kron_vbatched_plain_f64(int d, const int *m, const int *n,
+                                    const double *const *Aarray,
+                                    const double *const *pX, double *const *pY,
+                                    double *W, size_t Wcapacity_bytes, int batchCount,
+                                    kron_stream_t stream);
+kron_status kron_vbatched_plain_f32(int d, const int *m, const int *n,
+                                    const float *const *Aarray,
+                                    const float *const *pX, float *const *pY,
+                                    float *W, size_t Wcapacity_bytes, int batchCount,
+                                    kron_stream_t stream);
+
+/*
  * xbatched plan (no counterpart in the reference; the reference re-derives
  * nothing between calls).  For repeated applies with an unchanged pointer
  * map, plan_create validates the arguments of kron_xbatched_* and computes,

@@ -22,6 +22,7 @@ import os
 __all__ = [
     "KronError", "lib", "ptr_array", "XbatchedPlan", "MixedPlan", "MixedClass",
     "kronmult_batched", "kronmult_batched_adjoint", "kronmult_xbatched", "kronmult_vbatched",
+    "kronmult_vbatched_plain",
     "kgemm_nn_batched", "kgemm_nt_batched",
 ] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
 
@@ -46,6 +47,8 @@ _SIGS = {
     "kron_batched_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp],
     "kron_xbatched_%s": [_c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_vp],
     "kron_vbatched_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int, _c_vp],
+    "kron_vbatched_plain_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int,
+                               _c_vp],
     "kron_batched_adjoint_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int,
                                 _c_vp],
 }
@@ -191,6 +194,22 @@ def kronmult_vbatched(m, n, Aarray, Xarray, Yarray, W, Wcapacity, batchCount,
     mm = (ctypes.c_int * d)(*m)
     nn = (ctypes.c_int * d)(*n)
     _check("kronmult%d_vbatched" % d,
+           f(d, mm, nn, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(W), Wcapacity,
+             batchCount, _stream(stream)))
+
+
+def kronmult_vbatched_plain(m, n, Aarray, Xarray, Yarray, W, Wcapacity, batchCount,
+                            dtype=None, stream=None):
+    """kron_vbatched_plain_*: the kronmult_vbatched update with the reference's
+    per-entry structure (one workgroup per entry, one pass through W per mode);
+    the baseline the mixed-shape plan is compared with."""
+    import torch
+    d = len(m)
+    sfx = _sfx(dtype if dtype is not None else torch.float64)
+    f = getattr(lib(), "kron_vbatched_plain_" + sfx)
+    mm = (ctypes.c_int * d)(*m)
+    nn = (ctypes.c_int * d)(*n)
+    _check("kron_vbatched_plain",
            f(d, mm, nn, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(W), Wcapacity,
              batchCount, _stream(stream)))
 

@@ -164,7 +164,7 @@ static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
 template <typename T>
 static kron_status vbatched_impl(int d, const int *m, const int *n, const T *const *Aarray,
                                  const T *const *pX, T *const *pY, T *W, size_t Wcap,
-                                 int batch, kron_stream_t stream)
+                                 int batch, kron_stream_t stream, bool plain = false)
 {
     if (d < 1 || d > kMaxDim || batch < 0 || !m || !n) return KRON_EINVAL;
     int mm[kMaxDim], nn[kMaxDim];
@@ -196,6 +196,9 @@ static kron_status vbatched_impl(int d, const int *m, const int *n, const T *con
     long long subbatch = (long long)(Wcap / slot_bytes);
     if (subbatch > batch) subbatch = batch;
     if (subbatch < 1) subbatch = 1;
+    if (plain)
+        return launch_plain<T>(d, mm, nn, Aarray, pX, pY, W, sizeXW, (int)subbatch, batch,
+                               static_cast<hipStream_t>(stream));
     KronArgs<T> a;
     a.d = d;
     for (int k = 0; k < kMaxDim; ++k) {
@@ -376,6 +379,19 @@ kron_status kron_vbatched_f32(int d, const int *m, const int *n, const float *co
                               size_t Wcap, int batch, kron_stream_t s)
 {
     return vbatched_impl<float>(d, m, n, A, pX, pY, W, Wcap, batch, s);
+}
+
+kron_status kron_vbatched_plain_f64(int d, const int *m, const int *n, const double *const *A,
+                                   const double *const *pX, double *const *pY, double *W,
+                                   size_t Wcap, int batch, kron_stream_t s)
+{
+    return vbatched_impl<double>(d, m, n, A, pX, pY, W, Wcap, batch, s, true);
+}
+kron_status kron_vbatched_plain_f32(int d, const int *m, const int *n, const float *const *A,
+                                   const float *const *pX, float *const *pY, float *W,
+                                   size_t Wcap, int batch, kron_stream_t s)
+{
+    return vbatched_impl<float>(d, m, n, A, pX, pY, W, Wcap, batch, s, true);
 }
 
 kron_status kron_xbatched_plan_create_f64(kron_xplan *plan, int d, int n,

@@ -105,6 +105,14 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
 template <typename T>
 size_t adjoint_scratch_elems(int d, int n);
 
+// The reference's per-entry vbatched structure (kron_plain.hip): one
+// workgroup per entry, every mode a pass through the entry's scratch slot of
+// 2*slot elements in W, nslots entries per launch.
+template <typename T>
+kron_status launch_plain(int d, const int *m, const int *n, const T *const *pA,
+                         const T *const *pX, T *const *pY, T *W, long long slot, int nslots,
+                         int batch, hipStream_t stream);
+
 // Per-thread record of the last HIP error (kron_last_hip_error()).
 void set_last_hip_error(int code);
 int max_lds_bytes();

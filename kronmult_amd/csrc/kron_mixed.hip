@@ -91,7 +91,9 @@ struct MixPath {
     // on-chip classes: variant v halves the threads per entry v times
     static constexpr int G = BIG ? kMixThreads : (G0 >> V) > 0 ? (G0 >> V) : 1;
     static constexpr int P = kMixThreads / G;
-    static constexpr MixSplit SP = mix_split(D, N, L, BIG ? V : 0);
+    // elements of a two-pass tile
+    static constexpr int LT = L;
+    static constexpr MixSplit SP = mix_split(D, N, LT, BIG ? V : 0);
     static constexpr bool VALID =
         BIG ? SP.cost >= 0 : ((G0 >> V) >= 1 && (size_t)P * E <= (size_t)L);
     static constexpr size_t lds_bytes()
@@ -109,95 +111,383 @@ struct MixPath {
 // live than a single-shape kernel does.
 constexpr int kMixSBytes = 128;
 
-// One entry per workgroup, the entry resident in LDS (the fast kernel's mode
-// order: 2 from HBM, D..3 in place, 1 into Y).  The entries of the item share
-// one Y target when the class updates Y without atomics; they run in order,
-// and each thread updates the same Y elements every time.
-template <typename T, typename Acc, int D, int N>
-__device__ __forceinline__ void mix_onchip(const MixedClass<T> &c, const int *order, int count,
-                                           T *smem, int tid)
+// Paths with N <= 6 and at most 24 first-pass values per thread keep the
+// next unit's input in flight through their remaining passes (factor plus
+// input fit the 256 VGPRs of two waves per SIMD); the others stream their
+// first pass in batches.
+template <int N>
+constexpr bool mix_pipelined(int regs)
 {
-    const cptr<int> ord = to_const(order);
-    const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
-    const bool atomic = c.atomic_y != 0;
-    Acc a[N][N];
-    auto L = [&](int i) -> T & { return smem[lds_index<D, N>(i)]; };
-    for (int i = 0; i < count; ++i) {
-        const int b = ord[i];
-        int g = tid;
-        asm volatile("" : "+v"(g));
-        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 1), c.lda);
-        constexpr int F = cpow(N, D - 1);
-        const auto x = ent.template x_g<true>(b);
-        tile_pass_in<T, Acc, N, cpow(N, D - 2), F, kMixThreads>(
-            g, a, [&](int e) { return x[e]; }, [&](int e, Acc v) { L(e) = T(v); });
-        __syncthreads();
-        for_modes_desc<D, 3>([&](auto kc) {
-            constexpr int K = decltype(kc)::value;
-            load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, K - 1), c.lda);
-            fiber_pass<Acc, D, N, K, kMixThreads>(
-                g, a, [&](int e) { return L(e); }, [&](int e, Acc v) { L(e) = T(v); });
-            __syncthreads();
-        });
-        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0), c.lda);
-        const auto y = ent.template y_g<true>(b);
-        tile_pass_y<T, Acc, N, F, F, kMixThreads>(
-            g, a, [&](int e) { return L(e); }, y, [](int e) { return e; },
-            [](int) { return true; }, atomic);
-        __syncthreads();
+    return N <= 6 && regs <= 24;
+}
+
+// Launch state shared by the paths.
+template <typename T>
+struct MixCtx {
+    const MixedClass<T> *cls;
+    const MixedItem *items;
+    const MixedEntry *recs;  // per-entry pointers, in item order
+    int nitems;
+    unsigned *queue;
+    T *scratch;
+    T *smem;
+    int *s_next;     // LDS slot broadcasting a dequeued index
+    unsigned ahead;  // thread 0: the index requested for the item after next
+};
+
+// The next item index (workgroup-uniform).  Thread 0 keeps one request in
+// flight, so the atomic's round trip overlaps the current item.
+template <typename T>
+__device__ __forceinline__ int mix_dequeue(MixCtx<T> &x)
+{
+    if (threadIdx.x == 0) {
+        *x.s_next = (int)x.ahead;
+        x.ahead = atomicAdd(x.queue, 1u);
+    }
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*x.s_next);
+    __syncthreads();  // s_next is rewritten by the next dequeue
+    return it;
+}
+
+// Entry r's factor k, X and Y from its record: with r uniform, scalar loads
+// of one 64-byte record; the _v forms take a per-lane r.
+template <typename T>
+__device__ __forceinline__ cptr<T> mix_factor(const MixCtx<T> &x, int r, int k)
+{
+    return to_const(static_cast<const T *>((to_const(x.recs) + r)->A[k]));
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> mix_x(const MixCtx<T> &x, int r)
+{
+    return to_global(static_cast<const T *>((to_const(x.recs) + r)->X));
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> mix_y(const MixCtx<T> &x, int r)
+{
+    return to_global(static_cast<T *>((to_const(x.recs) + r)->Y));
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> mix_factor_v(const MixCtx<T> &x, int r, int k)
+{
+    return to_global(static_cast<const T *>((to_global(x.recs) + r)->A[k]));
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> mix_x_v(const MixCtx<T> &x, int r)
+{
+    return to_global(static_cast<const T *>((to_global(x.recs) + r)->X));
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> mix_y_v(const MixCtx<T> &x, int r)
+{
+    return to_global(static_cast<T *>((to_global(x.recs) + r)->Y));
+}
+
+// A work item with its class, read through scalar loads.
+template <typename T>
+struct MixUnit {
+    MixedClass<T> c;
+    int first, count, chunk;
+};
+
+template <typename T>
+__device__ __forceinline__ void mix_read(const MixCtx<T> &x, int it, MixUnit<T> &u)
+{
+    const auto ip = to_const(x.items) + it;
+    u.first = ip->first;
+    u.count = ip->count;
+    u.chunk = ip->chunk;
+    const auto cp = to_const(x.cls) + ip->cls;
+    u.c.pA = cp->pA;
+    u.c.pX = cp->pX;
+    u.c.pY = cp->pY;
+    u.c.d = cp->d;
+    u.c.n = cp->n;
+    u.c.lda = cp->lda;
+    u.c.path = cp->path;
+    u.c.atomic_y = cp->atomic_y;
+    u.c.wofs = cp->wofs;
+}
+
+template <typename T>
+__device__ __forceinline__ int mix_path_of(const MixCtx<T> &x, int it)
+{
+    if (it >= x.nitems) return -1;
+    const auto ip = to_const(x.items) + it;
+    return (to_const(x.cls) + ip->cls)->path;
+}
+
+// Runs units of one path back to back.  A unit is one entry of an item
+// (Path::PER_ENTRY) or the whole item.  The first-pass input of the next unit
+// -- the item's next entry, or the next queue item when it uses the same
+// path -- is loaded into the registers the current unit's first pass has
+// just consumed, so its HBM latency overlaps the current unit's remaining
+// passes (the fast kernel's pipeline, across work items).  Returns the next
+// item index of another path (or >= nitems).
+template <typename T, typename Path>
+__device__ __forceinline__ int mix_pipeline(MixCtx<T> &x, int it, int code, int tid)
+{
+    MixUnit<T> u;
+    mix_read(x, it, u);
+    int k = 0;
+    typename Path::Regs r;
+    // tid is made opaque again in every iteration: otherwise the address math
+    // of all passes is hoisted out of the unit loop and held in VGPRs
+    if constexpr (!Path::PIPE) {
+        // registers too tight to keep a prefetch in flight: one unit at a time
+        for (;;) {
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            Path::direct(x, u, k, t);
+            Path::rest(x, u, k, t);
+            if (Path::PER_ENTRY && k + 1 < u.count) {
+                ++k;
+                continue;
+            }
+            it = mix_dequeue(x);
+            if (mix_path_of(x, it) != code) return it;
+            mix_read(x, it, u);
+            k = 0;
+        }
+    }
+    Path::load(r, x, u, k, tid);
+    for (;;) {
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        MixUnit<T> nu = u;
+        int nk = k + 1, nit = it;
+        bool same = true;
+        if (!Path::PER_ENTRY || nk >= u.count) {
+            nit = mix_dequeue(x);
+            same = mix_path_of(x, nit) == code;
+            if (same) mix_read(x, nit, nu);
+            nk = 0;
+        }
+        Path::first(r, x, u, k, t);
+        if (same) Path::load(r, x, nu, nk, t);
+        Path::rest(x, u, k, t);
+        if (!same) return nit;
+        u = nu;
+        k = nk;
+        it = nit;
     }
 }
 
-// P = 256 / G entries per workgroup, G threads each (small entries); Y is
-// updated atomically (entries of one item may share Y).
-template <typename T, typename Acc, int D, int N, int G>
-__device__ __forceinline__ void mix_packed(const MixedClass<T> &c, const int *order, int count,
-                                           T *smem, int tid)
-{
-    constexpr int E = cpow(N, D);
-    const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
-    const int slot = tid / G;
-    const bool active = slot < count;
-    const int b = active ? to_global(order)[slot] : 0;
-    const int g = tid % G;
-    Acc a[N][N];
-    if constexpr (D == 1) {
-        if (active) {
-            load_factor<T, Acc, N, false, false>(a, ent.template factor_g<false>(b, 0), c.lda);
-            const auto x = ent.template x_g<false>(b);
-            const auto y = ent.template y_g<false>(b);
-            fiber_pass<Acc, 1, N, 1, G>(
-                g, a, [&](int e) { return x[e]; }, [&](int e, Acc v) { atomic_add(y + e, T(v)); });
+// One entry per workgroup, resident in LDS (the fast kernel's mode order:
+// 2 from HBM, D..3 in place, 1 into Y).  An item's entries share one Y target
+// when the class updates Y without atomics; they run in order and each
+// thread updates the same Y elements every time.
+template <typename T, typename Acc, int D, int N>
+struct PathOnchip {
+    static constexpr bool PER_ENTRY = true;
+    static constexpr int F = cpow(N, D - 1);
+    static constexpr int S2 = cpow(N, D - 2);
+    static constexpr int IT = (F + kMixThreads - 1) / kMixThreads;
+    static constexpr bool PIPE = mix_pipelined<N>(IT * N);
+    struct Regs {
+        T v[IT][N];
+    };
+    static __device__ __forceinline__ T &L(const MixCtx<T> &x, int i)
+    {
+        return x.smem[lds_index<D, N>(i)];
+    }
+    static __device__ __forceinline__ void load(Regs &rg, const MixCtx<T> &x,
+                                                const MixUnit<T> &u, int k, int tid)
+    {
+        const int r = u.first + k;
+        const auto xp = mix_x<T>(x, r);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = tid + it * kMixThreads;
+            if ((F % kMixThreads) == 0 || f < F) {
+                const int base = (f % S2) + (f / S2) * (S2 * N);
+#pragma unroll
+                for (int j = 0; j < N; ++j) rg.v[it][j] = xp[base + j * S2];
+            }
         }
-    } else {
-        T *buf = smem + slot * E;
-        auto L = [&](int i) -> T & { return buf[lds_index<D, N>(i)]; };
-        if (active) {
-            load_factor<T, Acc, N, false, false>(a, ent.template factor_g<false>(b, 1), c.lda);
-            const auto x = ent.template x_g<false>(b);
-            fiber_pass<Acc, D, N, 2, G>(
-                g, a, [&](int e) { return x[e]; }, [&](int e, Acc v) { L(e) = T(v); });
+    }
+    static __device__ __forceinline__ void first(const Regs &rg, const MixCtx<T> &x,
+                                                 const MixUnit<T> &u, int k, int tid)
+    {
+        const int r = u.first + k;
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, 1),
+                                                        u.c.lda);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = tid + it * kMixThreads;
+            if ((F % kMixThreads) == 0 || f < F) {
+                const int base = (f % S2) + (f / S2) * (S2 * N);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    Acc s = a[i][0] * Acc(rg.v[it][0]);
+#pragma unroll
+                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                    L(x, base + i * S2) = T(s);
+                }
+            }
         }
+    }
+    // load + first without holding the whole input: batched loads
+    static __device__ __forceinline__ void direct(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                  int tid)
+    {
+        const int r = u.first + k;
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, 1),
+                                                        u.c.lda);
+        const auto xp = mix_x<T>(x, r);
+        tile_pass_in<T, Acc, N, S2, F, kMixThreads>(
+            tid, a, [&](int e) { return xp[e]; }, [&](int e, Acc v) { L(x, e) = T(v); });
+    }
+    static __device__ __forceinline__ void rest(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                int tid)
+    {
+        const int r = u.first + k;
+        Acc a[N][N];
         __syncthreads();
         for_modes_desc<D, 3>([&](auto kc) {
             constexpr int K = decltype(kc)::value;
-            if (active) {
-                load_factor<T, Acc, N, false, false>(a, ent.template factor_g<false>(b, K - 1),
-                                                     c.lda);
-                fiber_pass<Acc, D, N, K, G>(
-                    g, a, [&](int e) { return L(e); }, [&](int e, Acc v) { L(e) = T(v); });
-            }
+            load_factor<T, Acc, N, false, true, kMixSBytes>(
+                a, mix_factor<T>(x, r, K - 1), u.c.lda);
+            fiber_pass<Acc, D, N, K, kMixThreads>(
+                tid, a, [&](int e) { return L(x, e); }, [&](int e, Acc v) { L(x, e) = T(v); });
             __syncthreads();
         });
-        if (active) {
-            load_factor<T, Acc, N, false, false>(a, ent.template factor_g<false>(b, 0), c.lda);
-            const auto y = ent.template y_g<false>(b);
-            fiber_pass<Acc, D, N, 1, G>(
-                g, a, [&](int e) { return L(e); }, [&](int e, Acc v) { atomic_add(y + e, T(v)); });
-        }
-        __syncthreads();
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, 0),
+                                                        u.c.lda);
+        tile_pass_y<T, Acc, N, F, F, kMixThreads>(
+            tid, a, [&](int e) { return L(x, e); }, mix_y<T>(x, r),
+            [](int e) { return e; }, [](int) { return true; }, u.c.atomic_y != 0);
+        __syncthreads();  // LDS is rewritten by the next unit
     }
-}
+};
+
+// P = 256 / G entries per workgroup, G threads each (small entries, or
+// d = 1); the unit is the whole item.  Y is updated atomically (entries of
+// one item may share Y).
+template <typename T, typename Acc, int D, int N, int G>
+struct PathPacked {
+    static constexpr bool PER_ENTRY = false;
+    static constexpr int E = cpow(N, D);
+    static constexpr int F = E / N;
+    static constexpr int S2 = D >= 2 ? cpow(N, D - 2) : 1;
+    static constexpr int IT = (F + G - 1) / G;
+    static constexpr bool PIPE = false;
+    struct Regs {
+        T v[IT][N];
+    };
+    // this thread's slot entry (b < 0: no entry)
+    static __device__ __forceinline__ int slot_entry(const MixCtx<T> &x, const MixUnit<T> &u,
+                                                     int tid)
+    {
+        const int slot = tid / G;
+        return slot < u.count ? u.first + slot : -1;
+    }
+    static __device__ __forceinline__ void load(Regs &rg, const MixCtx<T> &x,
+                                                const MixUnit<T> &u, int, int tid)
+    {
+        const int b = slot_entry(x, u, tid);
+        if (b < 0) return;
+        const int g = tid % G;
+        const auto xp = mix_x_v<T>(x, b);
+        constexpr int FS = D >= 2 ? S2 : 1;  // stride of the first pass's fibers
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = g + it * G;
+            if ((F % G) == 0 || f < F) {
+                const int base = (f % FS) + (f / FS) * (FS * N);
+#pragma unroll
+                for (int j = 0; j < N; ++j) rg.v[it][j] = xp[base + j * FS];
+            }
+        }
+    }
+    static __device__ __forceinline__ void first(const Regs &rg, const MixCtx<T> &x,
+                                                 const MixUnit<T> &u, int, int tid)
+    {
+        const int b = slot_entry(x, u, tid);
+        const int g = tid % G;
+        T *buf = x.smem + (tid / G) * E;
+        Acc a[N][N];
+        if (b >= 0) {
+            load_factor<T, Acc, N, false, false>(a, mix_factor_v<T>(x, b, D >= 2 ? 1 : 0),
+                                                 u.c.lda);
+            constexpr int FS = D >= 2 ? S2 : 1;
+            const auto y = mix_y_v<T>(x, b);
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int f = g + it * G;
+                if ((F % G) == 0 || f < F) {
+                    const int base = (f % FS) + (f / FS) * (FS * N);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        Acc s = a[i][0] * Acc(rg.v[it][0]);
+#pragma unroll
+                        for (int j = 1; j < N; ++j)
+                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                        if constexpr (D == 1)
+                            atomic_add(y + (base + i * FS), T(s));
+                        else
+                            buf[lds_index<D, N>(base + i * FS)] = T(s);
+                    }
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void direct(const MixCtx<T> &x, const MixUnit<T> &u, int,
+                                                  int tid)
+    {
+        const int b = slot_entry(x, u, tid);
+        if (b < 0) return;
+        const int g = tid % G;
+        T *buf = x.smem + (tid / G) * E;
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, false>(a, mix_factor_v<T>(x, b, D >= 2 ? 1 : 0),
+                                             u.c.lda);
+        constexpr int FS = D >= 2 ? S2 : 1;
+        const auto xp = mix_x_v<T>(x, b);
+        const auto y = mix_y_v<T>(x, b);
+        // one fiber at a time: batching loads through per-slot (non-uniform)
+        // pointers would hold a 64-bit address per load
+        tile_pass<Acc, N, FS, F, G>(g, a, [&](int e) { return xp[e]; }, [&](int e, Acc v) {
+            if constexpr (D == 1)
+                atomic_add(y + e, T(v));
+            else
+                buf[lds_index<D, N>(e)] = T(v);
+        });
+    }
+    static __device__ __forceinline__ void rest(const MixCtx<T> &x, const MixUnit<T> &u, int,
+                                                int tid)
+    {
+        if constexpr (D >= 2) {
+            const int b = slot_entry(x, u, tid);
+                const int g = tid % G;
+            T *buf = x.smem + (tid / G) * E;
+            auto L = [&](int i) -> T & { return buf[lds_index<D, N>(i)]; };
+            Acc a[N][N];
+            __syncthreads();
+            for_modes_desc<D, 3>([&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                if (b >= 0) {
+                    load_factor<T, Acc, N, false, false>(
+                        a, mix_factor_v<T>(x, b, K - 1), u.c.lda);
+                    fiber_pass<Acc, D, N, K, G>(
+                        g, a, [&](int e) { return L(e); }, [&](int e, Acc v) { L(e) = T(v); });
+                }
+                __syncthreads();
+            });
+            if (b >= 0) {
+                load_factor<T, Acc, N, false, false>(a, mix_factor_v<T>(x, b, 0),
+                                                     u.c.lda);
+                const auto y = mix_y_v<T>(x, b);
+                fiber_pass<Acc, D, N, 1, G>(
+                    g, a, [&](int e) { return L(e); },
+                    [&](int e, Acc v) { atomic_add(y + e, T(v)); });
+            }
+            __syncthreads();
+        }
+    }
+};
 
 // A large entry in two passes through its own scratch vector W (n^d
 // elements; entry b of the class at scratch + wofs + b * n^d).  Pass 1
@@ -207,129 +497,278 @@ __device__ __forceinline__ void mix_packed(const MixedClass<T> &c, const int *or
 // W visible across the chip).  Tile elements past the end of the entry
 // compute on stale LDS and are not stored.
 template <typename T, typename Acc, int D, int N, int H, int C1>
-__device__ __forceinline__ void mix_pass1(const MixedClass<T> &c, int b, int chunk, T *smem,
-                                          T *scratch, int tid)
-{
-    constexpr int S = cpow(N, H), F = cpow(N, D - H);
-    constexpr int T1 = S * C1;
-    const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
-    const auto W = to_global(scratch + c.wofs + (long long)b * (S * F));
-    const auto x = ent.template x_g<true>(b);
-    const int c0 = chunk * C1;
-    Acc a[N][N];
-    auto L = [&](int i) -> T & { return smem[i]; };
-    const int g = tid;
+struct PathPass1 {
+    static constexpr bool PER_ENTRY = false;
+    static constexpr int S = cpow(N, H), F = cpow(N, D - H);
+    static constexpr int T1 = S * C1;
+    static constexpr int NF = T1 / N;  // fibers of each tile mode
+    static constexpr int IT = (NF + kMixThreads - 1) / kMixThreads;
+    static constexpr bool PIPE = mix_pipelined<N>(IT * N);
+    struct Regs {
+        T v[IT][N];
+    };
     // tile element t = s * C1 + c is entry element s * F + c0 + c
-    auto gidx = [&](int t) { return (t / C1) * F + c0 + t % C1; };
-    auto valid = [&](int t) { return c0 + t % C1 < F; };
-    if constexpr (H == 1) {
-        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0),
-                                                        c.lda);
-        tile_pass_in<T, Acc, N, C1, T1 / N, kMixThreads>(
-            g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
+    static __device__ __forceinline__ int gidx(int t, int c0) { return (t / C1) * F + c0 + t % C1; }
+    static __device__ __forceinline__ bool valid(int t, int c0) { return c0 + t % C1 < F; }
+    static __device__ __forceinline__ void load(Regs &rg, const MixCtx<T> &x,
+                                                const MixUnit<T> &u, int, int tid)
+    {
+        const int r = u.first;
+        const int c0 = u.chunk * C1;
+        const auto xp = mix_x<T>(x, r);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = tid + it * kMixThreads;
+            if ((NF % kMixThreads) == 0 || f < NF) {
+                const int base = (f % C1) + (f / C1) * (C1 * N);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const int t = base + j * C1;
+                    rg.v[it][j] = valid(t, c0) ? xp[gidx(t, c0)] : T(0);
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void first(const Regs &rg, const MixCtx<T> &x,
+                                                 const MixUnit<T> &u, int, int tid)
+    {
+        const int r = u.first;
+        const int c0 = u.chunk * C1;
+        const auto W = to_global(x.scratch + u.c.wofs + (long long)r * (S * F));
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, H - 1),
+                                                        u.c.lda);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = tid + it * kMixThreads;
+            if ((NF % kMixThreads) == 0 || f < NF) {
+                const int base = (f % C1) + (f / C1) * (C1 * N);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    Acc s = a[i][0] * Acc(rg.v[it][0]);
+#pragma unroll
+                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                    const int t = base + i * C1;
+                    if constexpr (H == 1) {
+                        if (valid(t, c0)) W[gidx(t, c0)] = T(s);
+                    } else {
+                        x.smem[t] = T(s);
+                    }
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void direct(const MixCtx<T> &x, const MixUnit<T> &u, int,
+                                                  int tid)
+    {
+        const int r = u.first;
+        const int c0 = u.chunk * C1;
+        const auto W = to_global(x.scratch + u.c.wofs + (long long)r * (S * F));
+        const auto xp = mix_x<T>(x, r);
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, H - 1),
+                                                        u.c.lda);
+        tile_pass_in<T, Acc, N, C1, NF, kMixThreads>(
+            tid, a, [&](int t) { return valid(t, c0) ? xp[gidx(t, c0)] : T(0); },
             [&](int t, Acc v) {
-                if (valid(t)) W[gidx(t)] = T(v);
-            });
-    } else {
-        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H - 1),
-                                                        c.lda);
-        tile_pass_in<T, Acc, N, C1, T1 / N, kMixThreads>(
-            g, a, [&](int t) { return valid(t) ? x[gidx(t)] : T(0); },
-            [&](int t, Acc v) { L(t) = T(v); });
-        __syncthreads();
-        for_modes_desc<H - 1, 2>([&](auto kc) {
-            constexpr int K = decltype(kc)::value;
-            load_factor<T, Acc, N, false, true, kMixSBytes>(
-                a, ent.template factor_g<true>(b, K - 1), c.lda);
-            tile_pass<Acc, N, cpow(N, H - K) * C1, T1 / N, kMixThreads>(
-                g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
-            __syncthreads();
-        });
-        load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, 0),
-                                                        c.lda);
-        tile_pass<Acc, N, cpow(N, H - 1) * C1, T1 / N, kMixThreads>(
-            g, a, [&](int t) { return L(t); },
-            [&](int t, Acc v) {
-                if (valid(t)) W[gidx(t)] = T(v);
+                if constexpr (H == 1) {
+                    if (valid(t, c0)) W[gidx(t, c0)] = T(v);
+                } else {
+                    x.smem[t] = T(v);
+                }
             });
     }
-    __syncthreads();  // LDS is rewritten by the next item
-}
+    static __device__ __forceinline__ void rest(const MixCtx<T> &x, const MixUnit<T> &u, int,
+                                                int tid)
+    {
+        if constexpr (H > 1) {
+            const int r = u.first;
+            const int c0 = u.chunk * C1;
+                const auto W = to_global(x.scratch + u.c.wofs + (long long)r * (S * F));
+            auto L = [&](int i) -> T & { return x.smem[i]; };
+            Acc a[N][N];
+            __syncthreads();
+            for_modes_desc<H - 1, 2>([&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                load_factor<T, Acc, N, false, true, kMixSBytes>(
+                    a, mix_factor<T>(x, r, K - 1), u.c.lda);
+                tile_pass<Acc, N, cpow(N, H - K) * C1, NF, kMixThreads>(
+                    tid, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
+                __syncthreads();
+            });
+            load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, 0),
+                                                            u.c.lda);
+            tile_pass<Acc, N, cpow(N, H - 1) * C1, NF, kMixThreads>(
+                tid, a, [&](int t) { return L(t); },
+                [&](int t, Acc v) {
+                    if (valid(t, c0)) W[gidx(t, c0)] = T(v);
+                });
+            __syncthreads();  // LDS is rewritten by the next unit
+        }
+    }
+};
 
-// Pass 2 of slab tile `chunk` for the entries of one item, in order (they
+// Pass 2 of slab tile -2 - chunk for the entries of one item, in order (they
 // share one Y target when the class updates Y without atomics, and this
 // workgroup owns that slab of it).
 template <typename T, typename Acc, int D, int N, int H, int C2>
-__device__ __forceinline__ void mix_pass2(const MixedClass<T> &c, const int *order, int count,
-                                          int chunk, T *smem, T *scratch, int tid)
-{
-    const cptr<int> ord = to_const(order);
-    constexpr int S = cpow(N, H), F = cpow(N, D - H);
-    constexpr int T2 = C2 * F;
-    const Entries<T, Layout::Ptr> ent{c.pA, c.pX, c.pY, D};
-    const bool atomic = c.atomic_y != 0;
-    const int r0 = chunk * C2;
-    const int nv = (S - r0 < C2 ? S - r0 : C2) * F;  // valid tile elements
-    Acc a[N][N];
-    auto L = [&](int i) -> T & { return smem[i]; };
-    for (int i = 0; i < count; ++i) {
-        const int b = ord[i];
-        int g = tid;
-        asm volatile("" : "+v"(g));
-        // tile element t = r * F + f is entry element r0 * F + t
-        const auto w = to_global(scratch + c.wofs + (long long)b * (S * F) + (long long)r0 * F);
-        const auto yt = ent.template y_g<true>(b) + (long long)r0 * F;
-        auto ok = [&](int t) { return t < nv; };
-        auto ident = [](int t) { return t; };
+struct PathPass2 {
+    static constexpr bool PER_ENTRY = true;
+    static constexpr int S = cpow(N, H), F = cpow(N, D - H);
+    static constexpr int T2 = C2 * F;
+    static constexpr int NF = T2 / N;
+    static constexpr int IT = (NF + kMixThreads - 1) / kMixThreads;
+    static constexpr bool PIPE = mix_pipelined<N>(IT * N) && D - H > 1;
+    struct Regs {
+        T v[IT][N];
+    };
+    static __device__ __forceinline__ int rows(const MixUnit<T> &u) { return (-2 - u.chunk) * C2; }
+    // valid tile elements (tile element t = r * F + f is entry element r0 * F + t)
+    static __device__ __forceinline__ int nvalid(int r0) { return (S - r0 < C2 ? S - r0 : C2) * F; }
+    static __device__ __forceinline__ void load(Regs &rg, const MixCtx<T> &x,
+                                                const MixUnit<T> &u, int k, int tid)
+    {
+        const int r = u.first + k;
+        const int r0 = rows(u), nv = nvalid(r0);
+        const auto w = to_global(x.scratch + u.c.wofs + (long long)r * (S * F) + (long long)r0 * F);
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int f = tid + it * kMixThreads;
+            if ((NF % kMixThreads) == 0 || f < NF) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const int t = f * N + j;
+                    rg.v[it][j] = t < nv ? w[t] : T(0);
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void first(const Regs &rg, const MixCtx<T> &x,
+                                                 const MixUnit<T> &u, int k, int tid)
+    {
+        const int r = u.first + k;
+        const int r0 = rows(u), nv = nvalid(r0);
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, D - 1),
+                                                        u.c.lda);
         if constexpr (D - H == 1) {
-            load_factor<T, Acc, N, false, true, kMixSBytes>(
-                a, ent.template factor_g<true>(b, D - 1), c.lda);
-            tile_pass_y<T, Acc, N, 1, T2 / N, kMixThreads>(
-                g, a, [&](int t) { return t < nv ? w[t] : T(0); }, yt, ident, ok, atomic);
+            // the only fast mode: straight into Y
+            const auto yt = mix_y<T>(x, r) + (long long)r0 * F;
+            const bool atomic = u.c.atomic_y != 0;
+            T yv[IT][N];
+            if (!atomic) {
+#pragma unroll
+                for (int it = 0; it < IT; ++it) {
+                    const int f = tid + it * kMixThreads;
+                    if ((NF % kMixThreads) == 0 || f < NF) {
+#pragma unroll
+                        for (int i = 0; i < N; ++i) {
+                            const int t = f * N + i;
+                            yv[it][i] = t < nv ? yt[t] : T(0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int f = tid + it * kMixThreads;
+                if ((NF % kMixThreads) == 0 || f < NF) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        Acc s = a[i][0] * Acc(rg.v[it][0]);
+#pragma unroll
+                        for (int j = 1; j < N; ++j)
+                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                        const int t = f * N + i;
+                        if (t < nv) {
+                            if (atomic)
+                                atomic_add(yt + t, T(s));
+                            else
+                                yt[t] = T(Acc(yv[it][i]) + s);
+                        }
+                    }
+                }
+            }
         } else {
-            load_factor<T, Acc, N, false, true, kMixSBytes>(
-                a, ent.template factor_g<true>(b, D - 1), c.lda);
-            tile_pass_in<T, Acc, N, 1, T2 / N, kMixThreads>(
-                g, a, [&](int t) { return t < nv ? w[t] : T(0); },
-                [&](int t, Acc v) { L(t) = T(v); });
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int f = tid + it * kMixThreads;
+                if ((NF % kMixThreads) == 0 || f < NF) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        Acc s = a[i][0] * Acc(rg.v[it][0]);
+#pragma unroll
+                        for (int j = 1; j < N; ++j)
+                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                        x.smem[f * N + i] = T(s);
+                    }
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void direct(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                  int tid)
+    {
+        const int r = u.first + k;
+        const int r0 = rows(u), nv = nvalid(r0);
+        const auto w = to_global(x.scratch + u.c.wofs + (long long)r * (S * F) + (long long)r0 * F);
+        Acc a[N][N];
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, D - 1),
+                                                        u.c.lda);
+        auto ld = [&](int t) { return t < nv ? w[t] : T(0); };
+        if constexpr (D - H == 1) {
+            tile_pass_y<T, Acc, N, 1, NF, kMixThreads>(
+                tid, a, ld, mix_y<T>(x, r) + (long long)r0 * F,
+                [](int t) { return t; }, [&](int t) { return t < nv; }, u.c.atomic_y != 0);
+        } else {
+            tile_pass_in<T, Acc, N, 1, NF, kMixThreads>(tid, a, ld,
+                                                        [&](int t, Acc v) { x.smem[t] = T(v); });
+        }
+    }
+    static __device__ __forceinline__ void rest(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                int tid)
+    {
+        if constexpr (D - H > 1) {
+            const int r = u.first + k;
+            const int r0 = rows(u), nv = nvalid(r0);
+                auto L = [&](int i) -> T & { return x.smem[i]; };
+            Acc a[N][N];
             __syncthreads();
             for_modes_desc<D - 1, H + 2>([&](auto kc) {
                 constexpr int K = decltype(kc)::value;
                 load_factor<T, Acc, N, false, true, kMixSBytes>(
-                    a, ent.template factor_g<true>(b, K - 1), c.lda);
-                tile_pass<Acc, N, cpow(N, D - K), T2 / N, kMixThreads>(
-                    g, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
+                    a, mix_factor<T>(x, r, K - 1), u.c.lda);
+                tile_pass<Acc, N, cpow(N, D - K), NF, kMixThreads>(
+                    tid, a, [&](int t) { return L(t); }, [&](int t, Acc v) { L(t) = T(v); });
                 __syncthreads();
             });
-            load_factor<T, Acc, N, false, true, kMixSBytes>(a, ent.template factor_g<true>(b, H),
-                                                            c.lda);
-            tile_pass_y<T, Acc, N, cpow(N, D - H - 1), T2 / N, kMixThreads>(
-                g, a, [&](int t) { return L(t); }, yt, ident, ok, atomic);
+            load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, H),
+                                                            u.c.lda);
+            tile_pass_y<T, Acc, N, cpow(N, D - H - 1), NF, kMixThreads>(
+                tid, a, [&](int t) { return L(t); },
+                mix_y<T>(x, r) + (long long)r0 * F, [](int t) { return t; },
+                [&](int t) { return t < nv; }, u.c.atomic_y != 0);
+            __syncthreads();  // LDS is rewritten by the next unit
         }
-        __syncthreads();  // LDS is rewritten next
     }
-}
+};
 
-// chunk: -1 for a whole-entry item, c >= 0 for pass-1 tile c, -2 - c for
-// pass-2 tile c of a large entry.
-template <typename T, typename Acc, int D, int N, int V>
-__device__ __forceinline__ void mix_run(const MixedClass<T> &c, const int *ord, int count,
-                                        int chunk, T *smem, T *scratch, int tid)
+// The paths of (D, N, variant); chunk selects pass 1 (>= 0) or pass 2 (<= -2)
+// of two-pass classes.
+template <typename T, int D, int N, int V>
+__device__ __forceinline__ int mix_run(MixCtx<T> &x, int it, int code, int tid)
 {
     using Pth = MixPath<T, D, N, V>;
     if constexpr (!Pth::VALID) {
-        return;
+        return mix_dequeue(x);
     } else if constexpr (Pth::BIG) {
-        if (chunk >= 0)
-            mix_pass1<T, Acc, D, N, Pth::SP.h, Pth::SP.c1>(c, to_const(ord)[0], chunk, smem,
-                                                           scratch, tid);
-        else
-            mix_pass2<T, Acc, D, N, Pth::SP.h, Pth::SP.c2>(c, ord, count, -2 - chunk, smem,
-                                                           scratch, tid);
+        if (to_const(x.items)[it].chunk >= 0)
+            return mix_pipeline<T, PathPass1<T, T, D, N, Pth::SP.h, Pth::SP.c1>>(x, it, code, tid);
+        return mix_pipeline<T, PathPass2<T, T, D, N, Pth::SP.h, Pth::SP.c2>>(x, it, code, tid);
     } else if constexpr (Pth::P == 1 && D >= 2) {
-        mix_onchip<T, Acc, D, N>(c, ord, count, smem, tid);
+        return mix_pipeline<T, PathOnchip<T, T, D, N>>(x, it, code, tid);
     } else {
-        mix_packed<T, Acc, D, N, Pth::G>(c, ord, count, smem, tid);
+        return mix_pipeline<T, PathPacked<T, T, D, N, Pth::G>>(x, it, code, tid);
     }
 }
 
@@ -346,55 +785,30 @@ constexpr int mix_code(int D, int N, int V) { return (D * 16 + N) * kMixVariants
 
 template <typename T>
 __global__ void __launch_bounds__(kMixThreads, 2)
-kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const int *order,
+kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const MixedEntry *recs,
                   int nitems, unsigned *queue, T *scratch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
     __shared__ int s_next;
-    T *smem = reinterpret_cast<T *>(kron_smem);
-    // the next item's index is requested one item ahead, so the atomic's
-    // round trip overlaps the current item
-    unsigned ahead = 0;
-    if (threadIdx.x == 0) ahead = atomicAdd(queue, 1u);
-    for (;;) {
-        if (threadIdx.x == 0) {
-            s_next = (int)ahead;
-            ahead = atomicAdd(queue, 1u);
-        }
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_next);
-        __syncthreads();  // s_next is rewritten by the next dequeue
-        if (it >= nitems) break;
-        // the item and its class, field by field through scalar loads
-        const auto ip = to_const(items) + it;
-        const int ci = ip->cls, count = ip->count, chunk = ip->chunk;
-        const int *ord = order + ip->first;
-        const auto cp = to_const(cls) + ci;
-        MixedClass<T> c;
-        c.pA = cp->pA;
-        c.pX = cp->pX;
-        c.pY = cp->pY;
-        c.d = cp->d;
-        c.n = cp->n;
-        c.lda = cp->lda;
-        c.path = cp->path;
-        c.atomic_y = cp->atomic_y;
-        c.wofs = cp->wofs;
-        // lane-dependent index math of every path derives from this per-item
-        // opaque copy, so none of it is hoisted out of the loop (where it
-        // would stay live across all paths)
+    MixCtx<T> x{cls, items, recs, nitems, queue, scratch,
+                reinterpret_cast<T *>(kron_smem), &s_next, 0u};
+    if (threadIdx.x == 0) x.ahead = atomicAdd(queue, 1u);
+    int it = mix_dequeue(x);
+    while (it < nitems) {
+        // lane-dependent index math of every path derives from this opaque
+        // copy, so none of it is hoisted out of the loop (where it would stay
+        // live across all paths)
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        switch (c.path) {
+        const int code = mix_path_of(x, it);
+        switch (code) {
 #define KRON_MIX_CASE_V(D, N, V)                                                              \
-    case mix_code(D, N, V):                                                                   \
-        mix_run<T, T, D, N, V>(c, ord, count, chunk, smem, scratch, tid);                     \
-        break;
+    case mix_code(D, N, V): it = mix_run<T, D, N, V>(x, it, code, tid); break;
 #define KRON_MIX_CASE(D, N) KRON_MIX_CASE_V(D, N, 0) KRON_MIX_CASE_V(D, N, 1)
             KRON_MIX_FOR_EACH(KRON_MIX_CASE)
 #undef KRON_MIX_CASE
 #undef KRON_MIX_CASE_V
-        default: break;
+        default: it = mix_dequeue(x); break;
         }
     }
     // every workgroup has taken its last (empty) items; the last one out
@@ -465,7 +879,7 @@ kron_status mixed_launch(const MixedLaunch &L, hipStream_t stream)
         }
     }
     hipLaunchKernelGGL(kron_mixed_kernel<T>, dim3((unsigned)L.grid), dim3(kMixThreads), L.lds,
-                       stream, static_cast<const MixedClass<T> *>(L.classes), L.items, L.order,
+                       stream, static_cast<const MixedClass<T> *>(L.classes), L.items, L.recs,
                        L.nitems, L.queue, static_cast<T *>(L.scratch));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

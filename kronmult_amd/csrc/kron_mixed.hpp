@@ -23,10 +23,18 @@ struct MixedClass {
     int path;            // compiled path code (mixed_path_code)
     int atomic_y;        // 1: Y updated with atomics
     int pad;
-    long long wofs;      // scratch offset of entry 0 (two-pass classes)
+    long long wofs;      // scratch of record r at wofs + r * n^d (two-pass classes)
 };
 
-// A work item: entries order[first .. first+count) of class cls (one entry
+// The pointers of one entry, gathered by the plan from the caller's pointer
+// arrays (one 64-byte record, read with scalar loads).
+struct MixedEntry {
+    const void *A[kMaxDim];
+    const void *X;
+    void *Y;
+};
+
+// A work item: entry records [first, first+count) of class cls (one entry
 // for a pass-1 tile).
 struct MixedItem {
     int cls;
@@ -52,7 +60,7 @@ struct MixPathInfo {
 struct MixedLaunch {
     const void *classes = nullptr;  // MixedClass<T>[]
     const MixedItem *items = nullptr;
-    const int *order = nullptr;
+    const MixedEntry *recs = nullptr;
     int nitems = 0;
     unsigned *queue = nullptr;      // two counters, zero between launches
     void *scratch = nullptr;        // per-entry scratch of two-pass classes

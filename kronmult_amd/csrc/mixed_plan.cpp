@@ -375,7 +375,8 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
     std::vector<MixedItem> items;
     std::vector<double> icost;
     std::vector<int> ilaunch;
-    std::vector<int> order;
+    std::vector<int> order;       // class-local entry index of each record
+    std::vector<int> order_cls;   // class of each record
     long long wtotal = 0;  // scratch elements of the two-pass classes
     std::vector<long long> wofs(nclass, 0);
     double load[2] = {0, 0};
@@ -387,6 +388,7 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         const auto &keys = ykeys[c];
         const int base = (int)order.size();
         order.insert(order.end(), ord.begin(), ord.end());
+        order_cls.insert(order_cls.end(), ord.size(), c);
         MixPathInfo info;
         mixed_path_info<T>(C.desc.d, C.desc.n, C.variant, info);
         auto add = [&](MixedItem it, double cost, int launch) {
@@ -419,7 +421,8 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         // a two-pass entry moves X in and W out in pass 1 (2 vectors), W and
         // Y in and Y out in pass 2 (3 vectors): its cost splits 2 : 3
         if (C.kind == KRON_MIXED_KIND_TWOPASS) {
-            wofs[c] = wtotal;
+            // record r of the class (r >= base) uses scratch wofs + r * n^d
+            wofs[c] = wtotal - (long long)base * ipow_ll(C.desc.n, C.desc.d);
             wtotal += (long long)C.desc.batch * ipow_ll(C.desc.n, C.desc.d);
             for (int i = 0; i < C.desc.batch; ++i)
                 for (int t = 0; t < info.tiles1; ++t)
@@ -483,7 +486,30 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
     };
     put(&dc, dcls.data(), sizeof(MixedClass<T>) * dcls.size());
     put(&di, sitems.data(), sizeof(MixedItem) * sitems.size());
-    put(&dord, order.data(), sizeof(int) * order.size());
+    // the entries' pointers, gathered once from the caller's pointer arrays
+    std::vector<MixedEntry> recs(order.size());
+    for (int c = 0; c < nclass; ++c) {
+        const MixedPlanCore::Cls &C = p->cls[c];
+        if (C.kind == KRON_MIXED_KIND_LAUNCH || C.desc.batch == 0) continue;
+        std::vector<const void *> hA((size_t)C.desc.batch * C.desc.d), hX(C.desc.batch);
+        if ((e = hipMemcpy(hA.data(), C.desc.pA, sizeof(void *) * hA.size(),
+                           hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(hX.data(), C.desc.pX, sizeof(void *) * hX.size(),
+                           hipMemcpyDeviceToHost)) != hipSuccess) {
+            set_last_hip_error((int)e);
+            return bail(KRON_EHIP);
+        }
+        for (size_t r = 0; r < order.size(); ++r) {
+            if (order_cls[r] != c) continue;
+            const int b = order[r];
+            MixedEntry &m = recs[r];
+            for (int k = 0; k < kMaxDim; ++k)
+                m.A[k] = k < C.desc.d ? hA[(size_t)b * C.desc.d + k] : nullptr;
+            m.X = hX[b];
+            m.Y = reinterpret_cast<void *>(ykeys[c][b]);
+        }
+    }
+    put(&dord, recs.data(), sizeof(MixedEntry) * recs.size());
     if (e == hipSuccess && (e = hipMalloc(&dq, 4 * sizeof(unsigned))) == hipSuccess) {
         p->keep(dq);
         e = hipMemset(dq, 0, 4 * sizeof(unsigned));
@@ -499,7 +525,7 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         MixedLaunch &L = p->L[l];
         L.classes = dc;
         L.items = static_cast<const MixedItem *>(di) + (l == 0 ? 0 : nA);
-        L.order = static_cast<const int *>(dord);
+        L.recs = static_cast<const MixedEntry *>(dord);
         L.nitems = l == 0 ? nA : (int)sitems.size() - nA;
         L.queue = static_cast<unsigned *>(dq) + 2 * l;
         L.lds = lds;

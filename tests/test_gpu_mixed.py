@@ -276,3 +276,30 @@ def test_mixed_sparse_grid_workload_shape():
     assert info["items"] > 0 and info["grid"] > 0
     plan.execute()
     _assert_pools(case.got(), case.expected(), exact=False, tol=1e-12)
+
+
+@pytest.mark.parametrize("m,n", [([3, 4], [4, 2]), ([2, 3, 4], [3, 2, 2]), ([4] * 6, [4] * 6),
+                                 ([1, 2, 1, 2, 1], [2, 1, 2, 1, 2])])
+def test_plain_per_entry_kernel(m, n):
+    """The reference-structured baseline (kron_vbatched_plain_*) against the
+    oracle's vbatched, rectangular shapes, shared Y, W room for 3 slots (so
+    several sub-batches run)."""
+    d, batch, ny = len(m), 40, 7
+    sx, sy = int(np.prod(n)), int(np.prod(m))
+    sizes = [int(np.prod(m[:k + 1]) * np.prod(n[k + 1:])) for k in range(d - 1)]
+    sxw = max([sx] + sizes)
+    A_pool = H.integers(batch * d * 64, 41, -2, 2).reshape(batch * d, 64)
+    for k in range(d):
+        A_pool[k::d, m[k] * n[k]:] = np.nan
+    X_pool = H.integers(batch * sx, 42, -4, 4).reshape(batch, sx)
+    Y0 = H.integers(ny * sy, 43, -8, 8).reshape(ny, sy)
+    y_map = (H.splitmix64(batch, 44) % np.uint64(ny)).astype(np.int64)
+    want = Y0.copy()
+    Wh = np.zeros(2 * sxw * batch)
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * d).reshape(batch, d), X_pool,
+                    np.arange(batch), want, y_map, Wh, Wh.nbytes, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.full((3 * 2 * sxw,), float("nan"), dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched_plain(m, n, ptrs(Ad, np.arange(batch * d)), ptrs(Xd, np.arange(batch)),
+                              ptrs(Yd, y_map), Wd, Wd.numel() * 8, batch)
+    assert np.array_equal(host(Yd), want)
