@@ -19,4 +19,10 @@ kron_status fast_ptr_f64(int D, int N, const Entries<double, Layout::Ptr> &e, in
 kron_status fast_ptr_f32(int D, int N, const Entries<float, Layout::Ptr> &e, int lda,
                          int batch, hipStream_t s, const GroupOrder *ord);
 
+// The MFMA large-n path (kron_mfma.hip), batched layout, n in {16, 32}.
+bool mfma_supported(int d, int n, bool *needs_w);
+template <typename T>
+kron_status launch_mfma(int d, int n, const Entries<T, Layout::Batched> &e, T *W, int batch,
+                        hipStream_t s);
+
 }  // namespace kron

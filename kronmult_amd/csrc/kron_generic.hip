@@ -501,6 +501,16 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = fast_dispatch(a, a.n[0], a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
+    // n >= 16: every mode a dense GEMM on the matrix cores
+    if (square && !a.trans && a.layout == Layout::Batched && a.lda <= a.n[0] &&
+        mfma_supported(a.d, a.n[0], nullptr)) {
+        Entries<T, Layout::Batched> e{a.A, a.X, a.Y, a.d, (long long)a.n[0] * a.n[0], 0, 0};
+        long long E = 1;
+        for (int k = 0; k < a.d; ++k) E *= a.n[0];
+        e.sx = e.sy = E;
+        kron_status st = launch_mfma<T>(a.d, a.n[0], e, a.W, a.batch, a.stream);
+        if (st != KRON_ENOTSUP) return st;
+    }
     GenShape sh{};
     if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
