@@ -346,6 +346,61 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_large_n(args):
+    """Large-n dense path: d=3 n=32 and d=4 n=16, 2,048 entries each, FP64 and
+    FP32 on the matrix cores (batched layout, A and X normal(0, 1/n), W the
+    reference's (n^d, batch) workspace).  FP32 error stated against the FP64
+    oracle on the first entries."""
+    import torch
+
+    import helpers as H
+    import kronmult_amd as K
+    import oracle
+
+    dev = "cuda:0"
+    stream = torch.cuda.current_stream()
+    peak = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s
+    rows = []
+    for d, n in ((3, 32), (4, 16)):
+        batch, E = 2048, n ** d
+        s = 1.0 / np.sqrt(n)
+        A64 = H.normal(batch * d * n * n, 501 + d, s)
+        X64 = H.normal(batch * E, 502 + d, s)
+        for dt, tdt in (("f64", torch.float64), ("f32", torch.float32)):
+            A = torch.from_numpy(A64).to(dev, tdt)
+            X = torch.from_numpy(X64).to(dev, tdt)
+            Y = torch.zeros(batch * E, dtype=tdt, device=dev)
+            W = torch.empty(batch * E, dtype=tdt, device=dev)
+            ms = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, W, batch, stream=stream),
+                             args.steps, args.warmup, stream)
+            w = 8 if dt == "f64" else 4
+            alg = w * (3 * E + d * n * n) * batch
+            flops = 2 * d * n ** (d + 1) * batch
+            row = {"d": d, "n": n, "dtype": dt, "batch": batch, "ms": ms,
+                   "entries_per_s": batch / (ms * 1e-3),
+                   "alg_gbs": alg / (ms * 1e-3) / 1e9,
+                   "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "tflops": flops / (ms * 1e-3) / 1e12,
+                   "mfma_frac": flops / (ms * 1e-3) / 1e12 / peak[dt]}
+            if dt == "f32":
+                # one apply from Y = 0 on 4 entries against the FP64 oracle
+                k = 4
+                Ys = torch.zeros(k * E, dtype=tdt, device=dev)
+                K.kronmult_batched(d, n, A[:k * d * n * n], X[:k * E], Ys, W[:k * E], k,
+                                   stream=stream)
+                torch.cuda.synchronize()
+                a32 = A64[:k * d * n * n].astype(np.float32).astype(np.float64)
+                x32 = X64[:k * E].astype(np.float32).astype(np.float64)
+                want = np.zeros(k * E)
+                oracle.batched(oracle.oracle_lib(), d, n, a32, x32.copy(), want,
+                               np.zeros(k * E), k)
+                row["f32_rel_err_vs_f64_oracle"] = H.rel_err(Ys.cpu().numpy(), want)
+            rows.append(row)
+            del A, X, Y, W
+    print(json.dumps({"workload": "large-n dense path (MFMA), batched layout",
+                      "cases": rows}), flush=True)
+
+
 def bench_copy(args):
     """Device-to-device copy of 2 GiB: the achievable HBM read+write rate."""
     import torch
@@ -378,6 +433,8 @@ def main():
         bench_batched(args)
     elif args.workload == "adjoint_d6n6":
         bench_adjoint(args)
+    elif args.workload == "large_n":
+        bench_large_n(args)
     elif args.workload == "mixed_sparse_grid":
         bench_mixed(args)
     elif args.workload == "copy":
