@@ -81,6 +81,53 @@ __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__res
     }
 }
 
+// adj_gram for a compile-time n: thread t owns row p = t % N of dA and the
+// columns part, part + PARTS, ... (part = t / N); per column it reads one G
+// value and N P values into N independent accumulators (N-way ILP, about
+// (N+1)/N LDS reads per FMA), the column index advancing without branches.
+// The row is then added into the LDS accumulator (about 256/N lanes per
+// address; sharing rows between fewer threads serialises these adds more).
+template <typename T, int N>
+__device__ __forceinline__ void adj_gram_rows(const T *__restrict__ G, const T *__restrict__ P,
+                                              int outer, int inner, double *dA)
+{
+    constexpr int PARTS = 256 / N;
+    const int t = threadIdx.x;
+    if (t >= PARTS * N) return;
+    const int p = t % N, part = t / N;
+    const int cols = outer * inner;
+    double acc[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) acc[r] = 0;
+    int o = part / inner, ii = part % inner;
+    const int d_o = PARTS / inner, d_ii = PARTS % inner;
+#pragma unroll 2
+    for (int c = part; c < cols; c += PARTS) {
+        const int base = (o * N) * inner + ii;
+        const double g = double(G[base + p * inner]);
+#pragma unroll
+        for (int r = 0; r < N; ++r) acc[r] = __builtin_fma(g, double(P[base + r * inner]), acc[r]);
+        ii += d_ii;
+        const int wrap = ii >= inner ? 1 : 0;
+        ii -= wrap * inner;
+        o += d_o + wrap;
+    }
+#pragma unroll
+    for (int r = 0; r < N; ++r) unsafeAtomicAdd(&dA[p + r * N], acc[r]);
+}
+
+// dA(p, r) += G P^T over outer x n x inner tensors: the row form for a
+// compile-time n, else the pair form.
+template <typename T, int SQ>
+__device__ __forceinline__ void adj_gram_any(const T *__restrict__ G, const T *__restrict__ P,
+                                             int outer, int n, int inner, double *dA)
+{
+    if constexpr (SQ > 0)
+        adj_gram_rows<T, SQ>(G, P, outer, inner, dA);
+    else
+        adj_gram<T>(G, P, outer, n, inner, dA);
+}
+
 // Elements of one work item's input held per thread per input stream while
 // the previous item is worked on (an LDS buffer is at most kAdjPer*256).
 constexpr int kAdjPer = 8;
@@ -224,7 +271,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             int g = h;
             for (int i = 0; i < h; ++i) {
                 const int outer = ipow(n, i), inner = ipow(n, h - 1 - i) * C;
-                adj_gram<T>(B(g), B(i), outer, n, inner, dAacc + i * n * n);
+                adj_gram_any<T, SQ>(B(g), B(i), outer, n, inner, dAacc + i * n * n);
                 const int gn = (g == h) ? h + 1 : h;
                 contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n, fac(i), n);
                 __syncthreads();
@@ -246,7 +293,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             int g = nf;
             for (int t = 0; t < nf; ++t) {
                 const int outer = C * ipow(n, t), inner = ipow(n, nf - 1 - t);
-                adj_gram<T>(B(g), B(t), outer, n, inner, dAacc + (h + t) * n * n);
+                adj_gram_any<T, SQ>(B(g), B(t), outer, n, inner, dAacc + (h + t) * n * n);
                 if (t + 1 < nf) {
                     const int gn = (g == nf) ? nf + 1 : nf;
                     contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n,
@@ -266,6 +313,14 @@ __global__ void __launch_bounds__(256) kron_fill_zero(T *p, long long count)
 {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
         p[i] = T(0);
+}
+
+// LDS per workgroup: half of what a workgroup may use (two workgroups per
+// CU, as the kernel's registers allow), at most 80 KB.
+static long long adjoint_lds_cap()
+{
+    return std::min<long long>(max_lds_bytes() / 2 >= 32768 ? max_lds_bytes() / 2 : 65536,
+                               81920);
 }
 
 // Chunking for an LDS budget of `budget` elements (after the dA accumulator).
@@ -293,7 +348,8 @@ static bool plan_adjoint(AdjShape &sh, long long budget)
         s.buf = (int)std::max<long long>((long long)s.C2 * s.ES, nf > 0 ? (long long)s.C1 * s.EF : 0);
         const long long iters = (nf > 0 ? 2 * ((s.ES + s.C1 - 1) / s.C1) : 0) +
                                 (s.EF + s.C2 - 1) / s.C2;
-        // favour column runs of at least 8 elements (coalescing of pass 2)
+        // favour column runs of at least 8 elements (pass 2 reads and writes
+        // rows of C2 consecutive elements: shorter rows waste HBM lines)
         const long long cost = iters * (s.C2 < 8 && s.C2 < s.EF ? 4 : 1);
         if (!found || cost < best) {
             found = true;
@@ -313,7 +369,7 @@ size_t adjoint_scratch_elems(int d, int n)
     AdjShape sh{d, n, 0, E, 0, 0, 0, 0, 0, 0, 0};
     const long long fixed = ((long long)d * n * n * 8 + 15) / 16 * 16 +
                             ((long long)d * n * n + 1) / 2 * 2 * (long long)sizeof(T);
-    const long long budget = (std::min(max_lds_bytes(), 65536) - fixed) / (long long)sizeof(T);
+    const long long budget = (adjoint_lds_cap() - fixed) / (long long)sizeof(T);
     if (budget <= 0 || !plan_adjoint(sh, budget)) return (size_t)-1;
     return (sh.h < d) ? (size_t)(2 * E) : 0;
 }
@@ -328,7 +384,7 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
     AdjShape sh{d, n, 0, E, 0, 0, 0, 0, 0, 0, 0};
     const long long fixed = ((long long)d * n * n * 8 + 15) / 16 * 16 +
                             ((long long)d * n * n + 1) / 2 * 2 * (long long)sizeof(T);
-    const long long lds_cap = std::min(max_lds_bytes(), 65536);
+    const long long lds_cap = adjoint_lds_cap();
     const long long budget = (lds_cap - fixed) / (long long)sizeof(T);
     if (budget <= 0 || !plan_adjoint(sh, budget)) return KRON_ENOTSUP;
     if (sh.h < d && !W) return KRON_EWORKSPACE;
