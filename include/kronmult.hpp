@@ -27,6 +27,17 @@
                                 T *Xarray_[], T *Yarray_[], T *Warray_[],             \
                                 int const batchCount, int const subbatchCount = 0);
 
+// kronmult6_pbatched.cpp:5-11: contiguous A (n,n,d,batch), pointer X/Y/W
+#define KRONMULT_DECL_PBATCHED(D, T)                                                  \
+    void kronmult##D##_pbatched(int const n, T const Aarray_[], T *Xarray_[],        \
+                                T *Yarray_[], T *Warray_[], int const batchCount);
+
+KRONMULT_DECL_PBATCHED(1, double) KRONMULT_DECL_PBATCHED(2, double)
+KRONMULT_DECL_PBATCHED(3, double) KRONMULT_DECL_PBATCHED(4, double)
+KRONMULT_DECL_PBATCHED(5, double) KRONMULT_DECL_PBATCHED(6, double)
+KRONMULT_DECL_PBATCHED(1, float) KRONMULT_DECL_PBATCHED(2, float)
+KRONMULT_DECL_PBATCHED(3, float) KRONMULT_DECL_PBATCHED(4, float)
+KRONMULT_DECL_PBATCHED(5, float) KRONMULT_DECL_PBATCHED(6, float)
 KRONMULT_DECL_BATCHED(1, double) KRONMULT_DECL_BATCHED(2, double)
 KRONMULT_DECL_BATCHED(3, double) KRONMULT_DECL_BATCHED(4, double)
 KRONMULT_DECL_BATCHED(5, double) KRONMULT_DECL_BATCHED(6, double)

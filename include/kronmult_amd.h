@@ -81,6 +81,23 @@ kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *
                                      int batchCount, kron_stream_t stream);
 
 /*
+ * Contiguous factors, pointer vectors.  Replaces kronmult{1..6}_pbatched
+ * (kronmult6_pbatched.cpp:5-26, kernel kronmult6_pbatched.hpp:14-81).
+ *   A: (n, n, d, batchCount) as in kron_batched_*
+ *   pX[b], pY[b], pW[b] -> vectors of n^d (device pointer arrays); pY entries
+ *   may coincide (atomic add, as the reference's last kgemm step);
+ *   pW[b] is entry b's scratch, used only when n^d does not fit on chip.
+ * X is not written (the reference may clobber it).  The device table of
+ * factor pointers is built in a library-owned buffer per (device, stream).
+ */
+kron_status kron_pbatched_f64(int d, int n, const double *A, const double *const *pX,
+                              double *const *pY, double *const *pW, int batchCount,
+                              kron_stream_t stream);
+kron_status kron_pbatched_f32(int d, int n, const float *A, const float *const *pX,
+                              float *const *pY, float *const *pW, int batchCount,
+                              kron_stream_t stream);
+
+/*
  * Pointer-array layout with shared Y.  Replaces kronmult{1..6}_xbatched
  * (kronmult6_xbatched.cpp:5-39, body kronmult_xbatched.hpp:19-157).
  *   Aarray[k + b*d] -> n x n factor k of entry b, leading dimension lda >= n

@@ -22,9 +22,10 @@ import os
 __all__ = [
     "KronError", "lib", "ptr_array", "XbatchedPlan", "MixedPlan", "MixedClass",
     "kronmult_batched", "kronmult_batched_adjoint", "kronmult_xbatched", "kronmult_vbatched",
-    "kronmult_vbatched_plain",
+    "kronmult_vbatched_plain", "kronmult_pbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
-] + ["kronmult%d_%s" % (d, k) for d in range(1, 7) for k in ("batched", "xbatched", "vbatched")]
+] + ["kronmult%d_%s" % (d, k) for d in range(1, 7)
+     for k in ("batched", "pbatched", "xbatched", "vbatched")]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkronmult_amd.so")
@@ -46,6 +47,7 @@ _c_ip = ctypes.POINTER(ctypes.c_int)
 _SIGS = {
     "kron_batched_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp],
     "kron_xbatched_%s": [_c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_vp],
+    "kron_pbatched_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp],
     "kron_vbatched_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int, _c_vp],
     "kron_vbatched_plain_%s": [_c_int, _c_ip, _c_ip, _c_vp, _c_vp, _c_vp, _c_vp, _c_sz, _c_int,
                                _c_vp],
@@ -163,6 +165,19 @@ def kronmult_batched_adjoint(d, n, Aarray, Xarray, dYarray, dXarray, dAarray, Wa
     _check("kronmult%d_batched_adjoint" % d,
            f(d, n, _addr(Aarray), _addr(Xarray), _addr(dYarray), _addr(dXarray),
              _addr(dAarray), _addr(Warray), batchCount, _stream(stream)))
+
+
+def kronmult_pbatched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, dtype=None,
+                      stream=None):
+    """kronmult{d}_pbatched (kronmult6_pbatched.cpp:5-26): A (n,n,d,batch)
+    contiguous, X/Y/W as device arrays of pointers (Y may be shared)."""
+    import torch
+    sfx = _sfx(dtype if dtype is not None else (
+        Aarray.dtype if hasattr(Aarray, "dtype") else torch.float64))
+    f = getattr(lib(), "kron_pbatched_" + sfx)
+    _check("kronmult%d_pbatched" % d,
+           f(d, n, _addr(Aarray), _addr(Xarray), _addr(Yarray), _addr(Warray), batchCount,
+             _stream(stream)))
 
 
 def kronmult_xbatched(d, n, Aarray, lda, Xarray, Yarray, Warray, batchCount,
@@ -369,13 +384,18 @@ def _bind(d):
         return kronmult_vbatched(list(mn[0::2]), list(mn[1::2]), *rest, dtype=dtype,
                                  stream=stream)
 
-    for f, kind in ((batched, "batched"), (xbatched, "xbatched"), (vbatched, "vbatched")):
+    def pbatched(n, Aarray, Xarray, Yarray, Warray, batchCount, dtype=None, stream=None):
+        return kronmult_pbatched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, dtype,
+                                 stream)
+
+    for f, kind in ((batched, "batched"), (xbatched, "xbatched"), (vbatched, "vbatched"),
+                    (pbatched, "pbatched")):
         f.__name__ = "kronmult%d_%s" % (d, kind)
         f.__doc__ = "kronmult%d_%s -- see kronmult_%s." % (d, kind, kind)
-    return batched, xbatched, vbatched
+    return batched, xbatched, vbatched, pbatched
 
 
 for _d in range(1, 7):
     (globals()["kronmult%d_batched" % _d], globals()["kronmult%d_xbatched" % _d],
-     globals()["kronmult%d_vbatched" % _d]) = _bind(_d)
+     globals()["kronmult%d_vbatched" % _d], globals()["kronmult%d_pbatched" % _d]) = _bind(_d)
 del _d

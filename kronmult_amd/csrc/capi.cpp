@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 
 #include "kron_internal.hpp"
@@ -54,16 +55,21 @@ long long persistent_blocks(const void *kernel, size_t lds)
 
 void *stream_workspace(hipStream_t stream, size_t bytes)
 {
+    return stream_workspace_slot(stream, bytes, 0);
+}
+
+void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot)
+{
     struct Buf {
         void *ptr = nullptr;
         size_t bytes = 0;
     };
     static std::mutex mu;
-    static std::map<std::pair<int, hipStream_t>, Buf> bufs;
+    static std::map<std::tuple<int, hipStream_t, int>, Buf> bufs;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lk(mu);
-    Buf &b = bufs[std::make_pair(dev, stream)];
+    Buf &b = bufs[std::make_tuple(dev, stream, slot)];
     if (b.bytes < bytes) {
         // the old buffer may still be in use by queued work on this stream
         if (b.ptr) {
@@ -116,6 +122,34 @@ static kron_status batched_impl(int d, int n, const T *A, const T *X, T *Y, T *W
     a.batch = batch;
     a.stream = static_cast<hipStream_t>(stream);
     if (kron_scratch_elems(a) > 0 && !W) return KRON_EWORKSPACE;
+    return launch_kron(a);
+}
+
+// kronmult6_pbatched.hpp:14-81: factor k of entry b at A + (b*d + k)*n*n,
+// X/Y/W through pointer arrays, Y possibly shared (atomic last step).
+template <typename T>
+static kron_status pbatched_impl(int d, int n, const T *A, const T *const *pX, T *const *pY,
+                                 T *const *pW, int batch, kron_stream_t stream)
+{
+    long long E = 0;
+    if (d < 1 || d > kMaxDim || n < 1 || batch < 0 || !ipow_ok(n, d, E)) return KRON_EINVAL;
+    if (batch == 0) return KRON_OK;
+    if (!A || !pX || !pY) return KRON_EINVAL;
+    KronArgs<T> a;
+    a.d = d;
+    for (int k = 0; k < d; ++k) a.m[k] = a.n[k] = n;
+    a.lda = n;
+    a.layout = Layout::Ptr;
+    a.pX = pX;
+    a.pY = pY;
+    a.pW = pW;
+    a.scratch = Scratch::PtrEntry;
+    a.y_shared = true;
+    a.batch = batch;
+    a.stream = static_cast<hipStream_t>(stream);
+    if (kron_scratch_elems(a) > 0 && !pW) return KRON_EWORKSPACE;
+    a.pA = factor_table<T>(A, (long long)n * n, (long long)d * batch, a.stream);
+    if (!a.pA) return KRON_EHIP;
     return launch_kron(a);
 }
 
@@ -381,6 +415,17 @@ kron_status kron_vbatched_f32(int d, const int *m, const int *n, const float *co
     return vbatched_impl<float>(d, m, n, A, pX, pY, W, Wcap, batch, s);
 }
 
+kron_status kron_pbatched_f64(int d, int n, const double *A, const double *const *pX,
+                              double *const *pY, double *const *pW, int batch, kron_stream_t s)
+{
+    return pbatched_impl<double>(d, n, A, pX, pY, pW, batch, s);
+}
+kron_status kron_pbatched_f32(int d, int n, const float *A, const float *const *pX,
+                              float *const *pY, float *const *pW, int batch, kron_stream_t s)
+{
+    return pbatched_impl<float>(d, n, A, pX, pY, pW, batch, s);
+}
+
 kron_status kron_vbatched_plain_f64(int d, const int *m, const int *n, const double *const *A,
                                    const double *const *pX, double *const *pY, double *W,
                                    size_t Wcap, int batch, kron_stream_t s)
@@ -539,7 +584,17 @@ static void report(const char *fn, kron_status st)
                kron_xbatched_##SFX(D, n, Aarray_, lda, Xarray_, Yarray_, Warray_,          \
                                    batchCount, subbatchCount, nullptr));                   \
     }
+#define DEF_PBATCHED(D, T, SFX)                                                            \
+    void kronmult##D##_pbatched(int const n, T const Aarray_[], T *Xarray_[],             \
+                                T *Yarray_[], T *Warray_[], int const batchCount)          \
+    {                                                                                      \
+        report("kronmult" #D "_pbatched",                                                  \
+               kron_pbatched_##SFX(D, n, Aarray_, Xarray_, Yarray_, Warray_, batchCount,   \
+                                   nullptr));                                              \
+    }
 #define DEF_ALL(T, SFX)                                                                    \
+    DEF_PBATCHED(1, T, SFX) DEF_PBATCHED(2, T, SFX) DEF_PBATCHED(3, T, SFX)                \
+    DEF_PBATCHED(4, T, SFX) DEF_PBATCHED(5, T, SFX) DEF_PBATCHED(6, T, SFX)                \
     DEF_BATCHED(1, T, SFX) DEF_BATCHED(2, T, SFX) DEF_BATCHED(3, T, SFX)                   \
     DEF_BATCHED(4, T, SFX) DEF_BATCHED(5, T, SFX) DEF_BATCHED(6, T, SFX)                   \
     DEF_XBATCHED(1, T, SFX) DEF_XBATCHED(2, T, SFX) DEF_XBATCHED(3, T, SFX)                \

@@ -48,7 +48,7 @@ struct ScratchDesc {
     Scratch kind;
     T *W;                 // BatchedW base
     long long wstride;    // BatchedW elements per entry
-    T *const *pW;         // PtrSlots / PtrSlot0
+    T *const *pW;         // PtrSlots / PtrSlot0 / PtrEntry
     T *Wbase;             // Linear
     long long slot_stride;
     __device__ __forceinline__ T *get(int slot, int b) const
@@ -58,6 +58,7 @@ struct ScratchDesc {
         case Scratch::PtrSlots: return pW[2 * slot];
         case Scratch::PtrSlot0: return pW[0];
         case Scratch::Linear: return Wbase + (long long)slot * slot_stride;
+        case Scratch::PtrEntry: return pW[b];
         default: return nullptr;
         }
     }
@@ -529,6 +530,10 @@ kron_status launch_kron(const KronArgs<T> &a)
             scr.pW = a.pW;
             grid = std::min(a.batch, a.scratch == Scratch::PtrSlot0 ? 1 : a.nslots);
             break;
+        case Scratch::PtrEntry:
+            if (!a.pW) return KRON_EWORKSPACE;
+            scr.pW = a.pW;
+            break;
         case Scratch::Linear:
             if (!a.Wbase || a.nslots < 1) return KRON_EWORKSPACE;
             scr.Wbase = a.Wbase;
@@ -560,6 +565,31 @@ kron_status launch_kron(const KronArgs<T> &a)
     return launch_generic_t<T, Layout::Ptr, true, false>(e, sh, scr, grid, a.batch, a.stream);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+kron_factor_table(const T *A, long long fstride, long long count, const T **out)
+{
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+        out[i] = A + i * fstride;
+}
+
+template <typename T>
+const T *const *factor_table(const T *A, long long fstride, long long count, hipStream_t s)
+{
+    const T **tab = static_cast<const T **>(
+        stream_workspace_slot(s, (size_t)count * sizeof(T *), 1));
+    if (!tab) return nullptr;
+    const long long blocks = std::min<long long>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(kron_factor_table<T>, dim3((unsigned)blocks), dim3(256), 0, s, A, fstride,
+                       count, tab);
+    if (hipGetLastError() != hipSuccess) return nullptr;
+    return tab;
+}
+
+template const double *const *factor_table<double>(const double *, long long, long long,
+                                                   hipStream_t);
+template const float *const *factor_table<float>(const float *, long long, long long,
+                                                 hipStream_t);
 template kron_status launch_kron<double>(const KronArgs<double> &);
 template kron_status launch_kron<float>(const KronArgs<float> &);
 template size_t kron_scratch_elems<double>(const KronArgs<double> &);

@@ -24,7 +24,8 @@ enum class Scratch : int {
     BatchedW = 1,  // W + b*sizeX (batched layout: W is (n^d, batch))
     PtrSlots = 2,  // pW[2*slot] (xbatched, batch > 1; kronmult_xbatched.hpp:125-126)
     PtrSlot0 = 3,  // pW[0]      (xbatched, batch == 1)
-    Linear = 4     // Wbase + slot*slot_stride (vbatched; kronmult_vbatched.hpp:184-185)
+    Linear = 4,    // Wbase + slot*slot_stride (vbatched; kronmult_vbatched.hpp:184-185)
+    PtrEntry = 5   // pW[b], one vector per entry (pbatched; kronmult6_pbatched.hpp:63-70)
 };
 
 // Entry order for merged (grouped) pointer-layout calls: perm[r] is the r-th
@@ -95,6 +96,14 @@ kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t w
 // grown on demand and kept for the life of the process.  Work on one stream
 // is ordered, so calls on the same stream may reuse it.  nullptr on failure.
 void *stream_workspace(hipStream_t stream, size_t bytes);
+// The same with an independent buffer per slot (0: the Y-target sort,
+// 1: pointer tables built on the device).
+void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot);
+
+// Device table out[i] = A + i * fstride, i < count (pbatched factor pointers),
+// in the stream's slot-1 workspace.  nullptr on failure.
+template <typename T>
+const T *const *factor_table(const T *A, long long fstride, long long count, hipStream_t s);
 
 // Adjoint of the batched layout (kron_adjoint.hip): dX = K^T dY and dA_i,
 // both overwritten.  W: 2*n^d elements per entry (or nullptr when

@@ -72,6 +72,8 @@ def ref_lib():
         if os.path.exists(REF_SO):
             L = ctypes.CDLL(REF_SO)
             _declare(L, "ref", False)
+            L.ref_pbatched_f64.argtypes = [_i, _i, _vp, _vp, _vp, _vp, _i]
+            L.ref_pbatched_f64.restype = _i
         _cache["ref"] = L
     return _cache["ref"]
 
@@ -122,6 +124,27 @@ def xbatched(L, d, n, A_pool, a_map, lda, X_pool, x_map, Y_pool, y_map, W_pool, 
     px, py, pw = _ptrs(X_pool, x_map), _ptrs(Y_pool, y_map), _ptrs(W_pool, w_map)
     st = f(d, n, pa, lda, px, py, pw, batch, subbatch, *_omp(L, use_omp))
     assert st == 0, st
+
+
+def pbatched(L, d, n, A, X_pool, x_map, Y_pool, y_map, batch, use_omp=True):
+    """kronmult{d}_pbatched (kronmult6_pbatched.hpp:14-81): A (batch, d, n*n)
+    contiguous, X and Y rows of pools through maps, one W vector per entry.
+    The reference runs its own pbatched (d = 4 excepted: that file does not
+    compile as shipped); the restatement runs the same per-entry recursion
+    through its xbatched (factor k of entry b is row b*d + k of A), whose only
+    extra step, copying X first, does not change any value."""
+    E = X_pool.shape[1]
+    if L is not oracle_lib():
+        W = np.zeros((batch, E), dtype=X_pool.dtype)
+        Xc = X_pool.copy()  # the reference may clobber X
+        st = L.ref_pbatched_f64(d, n, _p(np.ascontiguousarray(A)), _ptrs(Xc, x_map),
+                                _ptrs(Y_pool, y_map), _ptrs(W, np.arange(batch)), batch)
+        assert st == 0, st
+        return
+    A2 = np.ascontiguousarray(A).reshape(batch * d, -1)
+    W = np.zeros((max(2, 2 * batch), E), dtype=X_pool.dtype)
+    xbatched(L, d, n, A2, np.arange(batch * d).reshape(batch, d), n, X_pool, x_map, Y_pool,
+             y_map, W, np.arange(W.shape[0]), batch, 0, use_omp)
 
 
 def vbatched(L, m, n, A_pool, a_map, X_pool, x_map, Y_pool, y_map, W, Wcap_bytes, batch,

@@ -25,6 +25,9 @@ LLVM_LIB = "/opt/rocm/llvm/lib"  # clang's OpenMP runtime (libomp)
 REF_SOURCES = (
     [f"kronmult{d}_batched.cpp" for d in range(1, 7)]
     + [f"kronmult{d}_xbatched.cpp" for d in range(1, 7)]
+    # kronmult4_pbatched.cpp:3 is not valid C++ as shipped (a malformed
+    # preprocessing directive); d = 4 pbatched is left out of this library
+    + [f"kronmult{d}_pbatched.cpp" for d in (1, 2, 3, 5, 6)]
     + ["kgemm_nn_batched.cpp", "kgemm_nt_batched.cpp"]
 )
 

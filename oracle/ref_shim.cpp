@@ -42,6 +42,12 @@
                                 int const lda, double *Xarray_[],           \
                                 double *Yarray_[], double *Warray_[],        \
                                 int const batchCount, int const subbatchCount);
+#define DECL_PBATCHED(D)                                                     \
+    void kronmult##D##_pbatched(int const n, double const Aarray_[],        \
+                                double *Xarray_[], double *Yarray_[],        \
+                                double *Warray_[], int const batchCount);
+DECL_PBATCHED(1) DECL_PBATCHED(2) DECL_PBATCHED(3)
+DECL_PBATCHED(5) DECL_PBATCHED(6)
 DECL_BATCHED(1) DECL_BATCHED(2) DECL_BATCHED(3)
 DECL_BATCHED(4) DECL_BATCHED(5) DECL_BATCHED(6)
 DECL_XBATCHED(1) DECL_XBATCHED(2) DECL_XBATCHED(3)
@@ -107,6 +113,22 @@ int ref_batched_f32(int d, int n, const float *A, float *X, float *Y, float *W,
     case 4: kronmult4_batched<float>(n, A, X, Y, W, batch); return 0;
     case 5: kronmult5_batched<float>(n, A, X, Y, W, batch); return 0;
     case 6: kronmult6_batched<float>(n, A, X, Y, W, batch); return 0;
+    }
+    return 1;
+}
+
+// kronmult{1..6}_pbatched (kronmult6_pbatched.cpp:5-26): double only in the
+// reference's host wrappers.  kronmult4_pbatched.cpp does not compile as
+// shipped (line 3), so d = 4 returns 1 (unavailable).
+int ref_pbatched_f64(int d, int n, const double *A, double **pX, double **pY, double **pW,
+                     int batch)
+{
+    switch (d) {
+    case 1: kronmult1_pbatched(n, A, pX, pY, pW, batch); return 0;
+    case 2: kronmult2_pbatched(n, A, pX, pY, pW, batch); return 0;
+    case 3: kronmult3_pbatched(n, A, pX, pY, pW, batch); return 0;
+    case 5: kronmult5_pbatched(n, A, pX, pY, pW, batch); return 0;
+    case 6: kronmult6_pbatched(n, A, pX, pY, pW, batch); return 0;
     }
     return 1;
 }

@@ -53,6 +53,8 @@ def test_reference_signatures_exported(lib):
             want.append("kronmult%d_batched(int, %s const*, %s*, %s*, %s*, int)" % (d, T, T, T, T))
             want.append("kronmult%d_xbatched(int, %s const* const*, int, %s**, %s**, %s**, int, int)"
                         % (d, T, T, T, T))
+            want.append("kronmult%d_pbatched(int, %s const*, %s**, %s**, %s**, int)"
+                        % (d, T, T, T, T))
             mn = ", ".join(["int"] * (2 * d))
             want.append("kronmult%d_vbatched(%s, %s const* const*, %s**, %s**, %s*, unsigned long, int)"
                         % (d, mn, T, T, T, T))
@@ -67,7 +69,8 @@ def test_reference_signatures_exported(lib):
                              text=True, check=True).stdout
         ref_syms = set(l.split(" ", 2)[2] for l in out.splitlines() if " T " in l)
         for w in want:
-            if "double" in w and "vbatched" not in w:
+            # kronmult4_pbatched.cpp does not compile as shipped (not in oracle/_ref)
+            if "double" in w and "vbatched" not in w and "kronmult4_pbatched" not in w:
                 assert w in ref_syms, "reference exports a different signature: " + w
 
 

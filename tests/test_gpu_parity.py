@@ -400,3 +400,51 @@ def test_xbatched_plan(d, n, batch, ny):
     plan.close()
     want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, np.zeros((ny, n ** d)), y_map)
     assert np.array_equal(host(Yd), 3 * want)
+
+
+# --- pbatched ------------------------------------------------------------------
+
+@pytest.mark.parametrize("d,n", [(1, 5), (2, 8), (3, 4), (4, 6), (5, 3), (6, 4), (6, 6),
+                                 (3, 16)])
+def test_pbatched_shared_y_integer_bit_exact(d, n):
+    """kronmult{d}_pbatched: contiguous A, pointer X/Y/W, Y shared between
+    entries (atomic last step), integer inputs, against the oracle; X must not
+    be written; (6, 6) and (3, 16) go through the per-entry W vectors."""
+    batch, nx, ny = 40 if n ** d < 20000 else 6, 9, 5
+    E = n ** d
+    A = H.integers(batch * d * n * n, 31 + d, -2, 2).reshape(batch, d, n * n)
+    X_pool = H.integers(nx * E, 32 + d, -3, 3).reshape(nx, E)
+    Y0 = H.integers(ny * E, 33 + d, -5, 5).reshape(ny, E)
+    x_map = (H.splitmix64(batch, 34 + d) % np.uint64(nx)).astype(np.int64)
+    y_map = (H.splitmix64(batch, 35 + d) % np.uint64(ny)).astype(np.int64)
+    want = Y0.copy()
+    oracle.pbatched(O, d, n, A, X_pool, x_map, want, y_map, batch)
+    Ad, Xd, Yd = dev(A), dev(X_pool), dev(Y0)
+    Wd = torch.full((batch, E), float("nan"), dtype=torch.float64, device=DEV)
+    X_before = Xd.clone()
+    K.kronmult_pbatched(d, n, Ad, ptrs(Xd, x_map), ptrs(Yd, y_map), ptrs(Wd, np.arange(batch)),
+                        batch)
+    assert np.array_equal(host(Yd), want)
+    assert torch.equal(Xd, X_before), "X must not be written"
+
+
+def test_pbatched_f32_and_errors():
+    d, n, batch = 4, 5, 33
+    E = n ** d
+    A = H.uniform(batch * d * n * n, 41).reshape(batch, d, n * n).astype(np.float32)
+    X = H.uniform(batch * E, 42).reshape(batch, E).astype(np.float32)
+    want = np.zeros((batch, E))
+    oracle.pbatched(O, d, n, A.astype(np.float64), X.astype(np.float64), np.arange(batch), want,
+                    np.arange(batch), batch)
+    Ad, Xd = dev(A), dev(X)
+    Yd = torch.zeros((batch, E), dtype=torch.float32, device=DEV)
+    K.kronmult_pbatched(d, n, Ad, ptrs(Xd, np.arange(batch)), ptrs(Yd, np.arange(batch)), None,
+                        batch)
+    assert H.rel_err(host(Yd), want) < F32_TOL
+    with pytest.raises(K.KronError) as e:
+        K.kronmult_pbatched(6, 6, dev(np.zeros(6 * 36)), ptrs(Xd, [0]), ptrs(Yd, [0]), None, 1,
+                            dtype=torch.float64)
+    assert e.value.status == K.KRON_EWORKSPACE
+    with pytest.raises(K.KronError) as e:
+        K.kronmult_pbatched(7, 2, Ad, ptrs(Xd, [0]), ptrs(Yd, [0]), None, 1)
+    assert e.value.status == K.KRON_EINVAL

@@ -109,3 +109,35 @@ def test_kgemm_bit_exact(trans, beta, dtype):
                          [mm] * batch, batch)
             outs.append(C)
         assert np.array_equal(outs[0], outs[1]), (mm, nn, kk)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 6])
+def test_pbatched_disjoint_y_bit_exact_and_shared_y_integer(d):
+    """kronmult{d}_pbatched (kronmult6_pbatched.hpp:14-81) in the reference
+    against the restatement: the same per-entry recursion, so bit-identical
+    with private Y; exact with integer inputs when Y is shared.  (d = 4: the
+    reference's kronmult4_pbatched.cpp does not compile as shipped.)"""
+    for n in (2, 3, 4):
+        if n ** d > 8192:
+            continue
+        batch, E = 6, n ** d
+        A = H.uniform(batch * d * n * n, 400 + d * 10 + n).reshape(batch, d, n * n)
+        X = H.uniform(batch * E, 500 + d * 10 + n).reshape(batch, E)
+        outs = []
+        for L in (O, R):
+            Y = np.zeros((batch, E))
+            oracle.pbatched(L, d, n, A, X.copy(), np.arange(batch), Y, np.arange(batch), batch)
+            outs.append(Y)
+        assert np.array_equal(outs[0], outs[1]), (d, n)
+        # shared Y, distinct X: the reference uses X as workspace for d >= 3
+        # ("X is the input (but may be modified)", kronmult6_pbatched.hpp:5),
+        # so entries sharing an X vector are not valid input there
+        Ai = H.integers(batch * d * n * n, 600 + d, -2, 2).reshape(batch, d, n * n)
+        Xi = H.integers(batch * E, 700 + d, -3, 3).reshape(batch, E)
+        x_map, y_map = np.arange(batch), np.arange(batch) % 2
+        outs = []
+        for L in (O, R):
+            Y = np.zeros((2, E))
+            oracle.pbatched(L, d, n, Ai, Xi.copy(), x_map, Y, y_map, batch)
+            outs.append(Y)
+        assert np.array_equal(outs[0], outs[1]), (d, n)
