@@ -10,18 +10,23 @@
 // (kronmult6_batched.hpp:51-71).
 //
 // Streaming in three passes per entry, with the modes split into slow S =
-// [0, h) and fast F = [h, d) and the entry viewed as E_S x E_F:
-//   pass 1 (contiguous slabs, fixed slow index):  V = (I_S (x) A_F^T) dY
-//   pass 2 (column chunks, fixed fast indices):   the P chain of X gives
+// [0, h) and fast F = [h, d) and the entry viewed as E_S x E_F (rows x
+// columns, row-major):
+//   pass 1 (slab chunks: rows [s0, s0+C1)):       V = (I_S (x) A_F^T) dY
+//   pass 2 (column chunks: columns [s0, s0+C2)):  the P chain of X gives
 //           U = (A_S (x) I_F) X and P_i for i in S; the G chain of V gives
 //           dX = (A_S^T (x) I_F) V and G_i for i in S; grams -> dA_S
-//   pass 3 (slabs):  P chain of U and G chain of dY over the fast modes;
-//           grams -> dA_F
-// V and U go through the entry's scratch (2 * n^d elements); HBM traffic is
-// 8 vectors per entry (forward two-pass: 5).  Intermediates are recomputed,
-// never stored beyond V and U.  Each pass is one launch over (entry, chunk)
-// work items; dA partial sums are added atomically into the cleared dA.
+//   pass 3 (slab chunks):  P chain of U and G chain of dY over the fast
+//           modes; grams -> dA_F
+// V and U live in the entry's scratch (2 * n^d elements) in column-chunk-major
+// order: column chunk c (ES rows x its width, row-major) is contiguous, so
+// pass 2 reads V and writes U as contiguous runs and passes 1 and 3 move one
+// run per chunk and slab.  HBM traffic is 8 vectors per entry (forward
+// two-pass: 5).  Each pass is one launch over (entry, chunk) work items; a
+// thread's gram rows stay in registers while its workgroup works on one
+// entry and are then added into dA (cleared beforehand).
 #include <algorithm>
+#include <cmath>
 
 #include "kron_device.hpp"
 
@@ -30,9 +35,8 @@ namespace kron {
 struct AdjShape {
     int d, n, h;
     long long E, ES, EF;   // n^d, n^h, n^(d-h)
-    int C1, C2, C3;        // chunk sizes of passes 1, 2, 3
-    int buf;               // elements per LDS buffer
-    int nbuf;              // LDS buffers
+    int C1, C2, C3;        // pass-1 slab rows, pass-2 column-chunk width, pass-3 slab rows
+    int nck2;              // column chunks, ceil(EF / C2)
 };
 
 // n^k for the small uniform exponents of an entry's mode extents.
@@ -43,11 +47,21 @@ __device__ __forceinline__ int ipow(int n, int k)
     return p;
 }
 
+// Quotient of small non-negative integers through the FP32 reciprocal:
+// int((x + 1/2) * rcp(d)) == x / d for 0 <= x < 2^21.  (x + 1/2) / d lies at
+// least 1/(2d) from every integer, and the rounding of rcp (1 ulp) and of the
+// product moves it by less than (x + 1/2) * 2^-22 / d < 1/(2d).
+struct FastDiv {
+    float r;
+    int d;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return {__builtin_amdgcn_rcpf(float(d)), d}; }
+__device__ __forceinline__ int quot(int x, FastDiv f) { return int((float(x) + 0.5f) * f.r); }
+
 // dA(p, r) += sum_{o, ii} G[o, p, ii] * P[o, r, ii] over tensors viewed as
-// outer x n x inner (dA in LDS, column-major n x n, double accumulation).
-// Thread t takes the pair t % npair and every (t / npair)-th (o, ii)
-// position, so the lanes of a wave read a handful of broadcast addresses;
-// indices advance without division.
+// outer x n x inner (dA in LDS, column-major n x n, double accumulation), for
+// a runtime n.  Thread t takes the pair t % npair and every (t / npair)-th
+// (o, ii) position; indices advance without division.
 template <typename T>
 __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__restrict__ P,
                                          int outer, int n, int inner, double *dA)
@@ -81,231 +95,413 @@ __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__res
     }
 }
 
-// adj_gram for a compile-time n: thread t owns row p = t % N of dA and the
-// columns part, part + PARTS, ... (part = t / N); per column it reads one G
-// value and N P values into N independent accumulators (N-way ILP, about
-// (N+1)/N LDS reads per FMA), the column index advancing without branches.
-// The row is then added into the LDS accumulator (about 256/N lanes per
-// address; sharing rows between fewer threads serialises these adds more).
+// Row p = t % N of the gram G P^T over the columns (o, ii) of two
+// outer x N x inner chunks, added into the thread's accumulators: thread t
+// takes the columns part, part + PARTS, ... (part = t / N) and reads one G
+// value and N P values per column (N independent FMAs).
 template <typename T, int N>
-__device__ __forceinline__ void adj_gram_rows(const T *__restrict__ G, const T *__restrict__ P,
-                                              int outer, int inner, double *dA)
+__device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *__restrict__ P,
+                                             int outer, int inner, double (&acc)[N])
 {
     constexpr int PARTS = 256 / N;
     const int t = threadIdx.x;
     if (t >= PARTS * N) return;
     const int p = t % N, part = t / N;
     const int cols = outer * inner;
-    double acc[N];
-#pragma unroll
-    for (int r = 0; r < N; ++r) acc[r] = 0;
-    int o = part / inner, ii = part % inner;
-    const int d_o = PARTS / inner, d_ii = PARTS % inner;
+    const FastDiv di = fast_div(inner);
 #pragma unroll 2
     for (int c = part; c < cols; c += PARTS) {
+        const int o = quot(c, di), ii = c - o * inner;
         const int base = (o * N) * inner + ii;
         const double g = double(G[base + p * inner]);
 #pragma unroll
         for (int r = 0; r < N; ++r) acc[r] = __builtin_fma(g, double(P[base + r * inner]), acc[r]);
-        ii += d_ii;
-        const int wrap = ii >= inner ? 1 : 0;
-        ii -= wrap * inner;
-        o += d_o + wrap;
     }
-#pragma unroll
-    for (int r = 0; r < N; ++r) unsafeAtomicAdd(&dA[p + r * N], acc[r]);
 }
 
-// dA(p, r) += G P^T over outer x n x inner tensors: the row form for a
-// compile-time n, else the pair form.
-template <typename T, int SQ>
-__device__ __forceinline__ void adj_gram_any(const T *__restrict__ G, const T *__restrict__ P,
-                                             int outer, int n, int inner, double *dA)
+// One mode contraction of a chunk held in LDS as outer x N x inner:
+// dst(o, i, ii) = sum_j F(i, j) src(o, j, ii), F = A_k or A_k^T (column-major
+// in LDS).  Thread t contracts the fibers t, t + 256, ...; the factor is read
+// into VGPRs (a broadcast LDS read per value).
+template <typename T, int N, bool TRANS>
+__device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__restrict__ dst,
+                                             int outer, int inner, const T *__restrict__ Al)
 {
-    if constexpr (SQ > 0)
-        adj_gram_rows<T, SQ>(G, P, outer, inner, dA);
-    else
-        adj_gram<T>(G, P, outer, n, inner, dA);
+    const int fibers = outer * inner;
+    if ((int)threadIdx.x >= fibers) return;
+    T a[N][N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[i][j] = TRANS ? Al[j + i * N] : Al[i + j * N];
+    const FastDiv di = fast_div(inner);
+    for (int f = threadIdx.x; f < fibers; f += 256) {
+        const int o = quot(f, di), ii = f - o * inner;
+        const int base = (o * N) * inner + ii;
+        T in[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) in[j] = src[base + j * inner];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T acc = a[i][0] * in[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
+            dst[base + i * inner] = acc;
+        }
+    }
 }
 
 // Elements of one work item's input held per thread per input stream while
-// the previous item is worked on (an LDS buffer is at most kAdjPer*256).
+// the previous item is worked on (a chunk is at most kAdjPer * 256 elements).
 constexpr int kAdjPer = 8;
+constexpr int kAdjChunk = kAdjPer * 256;
 
-// One pass of the adjoint over work items (entry, chunk):
-//   phase 1: slab chunk   -> V = (I_S (x) A_F^T) dY
-//   phase 2: column chunk -> U, dX, dA_S partial sums
-//   phase 3: slab chunk   -> dA_F partial sums
-// Each workgroup walks a contiguous range of items with the next item's
-// inputs prefetched into registers; dA partial sums stay in LDS while the
-// entry does not change and are then added to dA (cleared beforehand).
-template <typename T, typename Acc, int SQ>
+// Position of element q (0 <= q < rows * EF, in scratch order) of the slab
+// rows [s0, s0 + rows) of an entry's column-chunk-major scratch: returns the
+// scratch offset; w is the element's index in the row-major slab (rows x EF).
+struct SlabMap {
+    FastDiv blk, full, last;  // rows * C2, C2, width of the last chunk
+    int EF, C2, nck2, s0;
+    long long ES;
+};
+__device__ __forceinline__ long long slab_map(const SlabMap &m, int q, int &w)
+{
+    const int c = quot(q, m.blk);
+    const int rem = q - c * m.blk.d;
+    const bool last = c == m.nck2 - 1;
+    const int Cc = last ? m.last.d : m.C2;
+    const int r = quot(rem, last ? m.last : m.full);
+    w = r * m.EF + c * m.C2 + (rem - r * Cc);
+    return m.ES * (long long)(c * m.C2) + (long long)m.s0 * Cc + rem;
+}
+
+// One pass of the adjoint (PHASE 1, 2 or 3) over work items (entry, chunk).
+// Each workgroup walks a contiguous range of items: it issues the next item's
+// loads, works on the current item in LDS, stages the next item's inputs into
+// LDS and only then issues the current item's stores, so no wait for loaded
+// data also waits for stores of the same item.  Input buffers are never the
+// buffers results are stored from.  N > 0 is the compile-time factor size
+// (thread gram rows in registers for N <= 6); N == 0 handles any n.
+template <typename T, int N, int PHASE>
 __global__ void __launch_bounds__(256)
 kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *__restrict__ dY,
                     T *__restrict__ dX, T *__restrict__ dA, T *__restrict__ W, AdjShape sh,
-                    int batch, int phase)
+                    int batch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
-    const int d = sh.d, n = sh.n, h = sh.h;
+    constexpr bool KEEP = N > 0 && N <= 6;  // gram rows kept across an entry's chunks
+    constexpr int GA = KEEP ? (PHASE == 2 ? kMaxDim : kMaxDim - 1) : 1;
+    constexpr int NR = N > 0 ? N : 1;
+    const int d = sh.d, n = N > 0 ? N : sh.n, h = sh.h, nf = d - h;
+    const int nn = n * n, ddn = d * nn;
     double *dAacc = reinterpret_cast<double *>(kron_smem);  // d*n*n
-    T *facL = reinterpret_cast<T *>(kron_smem + ((size_t)d * n * n * sizeof(double) + 15) / 16 * 16);
-    T *bufs = facL + ((size_t)d * n * n + 1) / 2 * 2;  // after the staged factors
-    auto B = [&](int i) { return bufs + (size_t)i * sh.buf; };
-    auto fac = [&](int k) { return facL + k * n * n; };
+    T *facL = reinterpret_cast<T *>(kron_smem + ((size_t)ddn * sizeof(double) + 15) / 16 * 16);
+    T *bufs = facL + (ddn + 1) / 2 * 2;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long long E = sh.E, ES = sh.ES, EF = sh.EF;
-    const int nf = d - h;  // number of fast modes
-    const int ddn = d * n * n;
-    const long long nck = (phase == 2) ? (EF + sh.C2 - 1) / sh.C2 : (ES + sh.C1 - 1) / sh.C1;
+    const int C = PHASE == 1 ? sh.C1 : PHASE == 2 ? sh.C2 : sh.C3;
+    const int bufsz = PHASE == 2 ? (int)ES * C : C * (int)EF;
+    auto B = [&](int i) __attribute__((always_inline)) { return bufs + (size_t)i * bufsz; };
+    auto fac = [&](int k) __attribute__((always_inline)) { return facL + k * nn; };
+    const int nck = PHASE == 2 ? sh.nck2 : (int)((ES + C - 1) / C);
     const long long items = (long long)batch * nck;
     const long long per = (items + gridDim.x - 1) / gridDim.x;
     const long long r0 = blockIdx.x * per, r1 = min(items, r0 + per);
+    // modes whose dA this pass accumulates
+    const int k0 = PHASE == 3 ? h : 0, ng = PHASE == 3 ? nf : h;
 
-    // item r -> entry, chunk start, chunk size, element count
-    auto decode = [&](long long r, int &b, long long &s0, int &C, int &cnt)
-                      __attribute__((always_inline)) {
-        b = (int)(r / nck);
-        const long long ck = r % nck;
-        if (phase == 2) {
-            s0 = ck * sh.C2;
-            C = (int)min((long long)sh.C2, EF - s0);
-            cnt = (int)ES * C;
+    // chunk geometry: rows (passes 1, 3) or columns (pass 2) [s0, s0 + Cx)
+    auto geom = [&](int ck, int &s0, int &Cx, int &cnt) __attribute__((always_inline)) {
+        s0 = ck * C;
+        if (PHASE == 2) {
+            Cx = (int)min((long long)C, EF - s0);
+            cnt = (int)ES * Cx;
         } else {
-            s0 = ck * sh.C1;
-            C = (int)min((long long)sh.C1, ES - s0);
-            cnt = C * (int)EF;
+            Cx = (int)min((long long)C, ES - s0);
+            cnt = Cx * (int)EF;
         }
     };
+    auto slab = [&](int s0, int rows) __attribute__((always_inline)) {
+        SlabMap m;
+        m.blk = fast_div(rows * sh.C2);
+        m.full = fast_div(sh.C2);
+        m.last = fast_div((int)(EF - (long long)(sh.nck2 - 1) * sh.C2));
+        m.EF = (int)EF;
+        m.C2 = sh.C2;
+        m.nck2 = sh.nck2;
+        m.s0 = s0;
+        m.ES = ES;
+        return m;
+    };
+
     T v0[kAdjPer], v1[kAdjPer];
-    auto load = [&](long long r) __attribute__((always_inline)) {
-        int b, C, cnt;
-        long long s0;
-        decode(r, b, s0, C, cnt);
-        // phase 1: dY slab; phase 2: X and V (dY when every mode is slow)
-        // columns; phase 3: U and dY slabs
-        const T *dy = dY + b * E;
-        const T *V = W + (long long)b * 2 * E;
-        const T *p0 = (phase == 1) ? dy : (phase == 2) ? X + b * E : V + E;
-        const T *p1 = (phase == 2 && nf > 0) ? V : dy;
-        if (phase != 2) {
-            p0 += s0 * EF;
-            p1 += s0 * EF;
-        }
-        const auto g0 = to_global(p0), g1 = to_global(p1);
+    // loads of item (b, ck) into v0 / v1:
+    //   pass 1: dY slab;  pass 2: X columns and V chunk (dY columns when every
+    //   mode is slow);  pass 3: U slab (scratch order) and dY slab
+    auto load = [&](int b, int ck) __attribute__((always_inline)) {
+        int s0, Cx, cnt;
+        geom(ck, s0, Cx, cnt);
+        const long long eb = (long long)b * E;
+        const auto gdy = to_global(dY + eb);
+        if constexpr (PHASE == 1) {
 #pragma unroll
-        for (int k = 0; k < kAdjPer; ++k) {
-            const int w = tid + k * 256;
-            if (w < cnt) {
-                const long long g = (phase == 2) ? (long long)(w / C) * EF + s0 + w % C : w;
-                v0[k] = g0[g];
-                if (phase != 1) v1[k] = g1[g];
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) v0[k] = gdy[(long long)s0 * EF + q];
+            }
+        } else if constexpr (PHASE == 2) {
+            const auto gx = to_global(X + eb);
+            const auto gv = to_global(W + 2 * eb + ES * s0);
+            const FastDiv dc = fast_div(Cx);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    const int is = quot(q, dc);
+                    const long long g = (long long)is * EF + s0 + (q - is * Cx);
+                    v0[k] = gx[g];
+                    v1[k] = nf > 0 ? gv[q] : gdy[g];
+                }
+            }
+        } else {
+            const auto gu = to_global(W + 2 * eb + E);
+            const SlabMap m = slab(s0, Cx);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    int w;
+                    v0[k] = gu[slab_map(m, q, w)];
+                    v1[k] = gdy[(long long)s0 * EF + q];
+                }
             }
         }
     };
-    auto flush = [&](int b) __attribute__((always_inline)) {
-        // this entry's share of dA: modes [0, h) in phase 2, [h, d) in phase 3
-        const int k0 = (phase == 2) ? 0 : h, k1 = (phase == 2) ? h : d;
-        const auto dAb = to_global(dA + (long long)b * ddn);
-        for (int w = k0 * n * n + tid; w < k1 * n * n; w += nthr) atomic_add(dAb + w, T(dAacc[w]));
+    // the loaded item into its input buffers: pass 1 B(0); pass 2 B(h-1) (X),
+    // B(h) (V); pass 3 B(nf-1) (U), B(nf) (dY)
+    auto stage = [&](int ck) __attribute__((always_inline)) {
+        int s0, Cx, cnt;
+        geom(ck, s0, Cx, cnt);
+        if constexpr (PHASE == 1) {
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) B(0)[q] = v0[k];
+            }
+        } else if constexpr (PHASE == 2) {
+            T *bx = B(h - 1), *bv = B(h);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    bx[q] = v0[k];
+                    bv[q] = v1[k];
+                }
+            }
+        } else {
+            T *bu = B(nf - 1), *by = B(nf);
+            const SlabMap m = slab(s0, Cx);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    int w;
+                    slab_map(m, q, w);
+                    bu[w] = v0[k];
+                    by[q] = v1[k];
+                }
+            }
+        }
     };
+
+    double acc[GA][NR];
+#pragma unroll
+    for (int g = 0; g < GA; ++g)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[g][r] = 0;
+    // gram g of this pass (mode k0 + g): G P^T of outer x n x inner chunks
+    auto gram = [&](const T *G, const T *P, int outer, int inner, auto gc)
+                    __attribute__((always_inline)) {
+        constexpr int g = decltype(gc)::value;
+        if constexpr (N == 0) {
+            adj_gram<T>(G, P, outer, n, inner, dAacc + (k0 + g) * nn);
+        } else if constexpr (KEEP) {
+            adj_gram_row<T, N>(G, P, outer, inner, acc[g]);
+        } else {
+            adj_gram_row<T, N>(G, P, outer, inner, acc[0]);
+            if (tid < (256 / N) * N) {
+                const int p = tid % N;
+#pragma unroll
+                for (int r = 0; r < N; ++r) {
+                    unsafeAtomicAdd(&dAacc[(k0 + g) * nn + p + r * N], acc[0][r]);
+                    acc[0][r] = 0;
+                }
+            }
+        }
+    };
+    auto contract = [&](const T *src, T *dst, int outer, int inner, int k, auto tr)
+                        __attribute__((always_inline)) {
+        constexpr bool TR = decltype(tr)::value;
+        if constexpr (N == 0)
+            contract_fibers<T, T, TR, 8>(src, dst, outer, n, inner, n, fac(k), n);
+        else
+            adj_contract<T, N, TR>(src, dst, outer, inner, fac(k));
+    };
+    // this entry's share of dA: registers -> LDS -> one atomic per element
+    auto flush = [&](int b) __attribute__((always_inline)) {
+        if constexpr (KEEP) {
+            if (tid < (256 / NR) * NR) {
+                const int p = tid % NR;
+#pragma unroll
+                for (int g = 0; g < GA; ++g)
+                    if (g < ng) {
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) {
+                            unsafeAtomicAdd(&dAacc[(k0 + g) * nn + p + r * NR], acc[g][r]);
+                            acc[g][r] = 0;
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+        const auto dAb = to_global(dA + (long long)b * ddn);
+        for (int w = k0 * nn + tid; w < (k0 + ng) * nn; w += nthr) {
+            atomic_add(dAb + w, T(dAacc[w]));
+            dAacc[w] = 0;
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using TRUE_ = std::integral_constant<bool, true>;
+    using FALSE_ = std::integral_constant<bool, false>;
 
     for (int w = tid; w < ddn; w += nthr) dAacc[w] = 0;
-    if (r0 < r1) load(r0);
+    int b = (int)(r0 / nck), ck = (int)(r0 % nck);
+    if (r0 < r1) load(b, ck);
     int cur_b = -1;
+    bool staged = false;
     for (long long r = r0; r < r1; ++r) {
-        int b, C, cnt;
-        long long s0;
-        decode(r, b, s0, C, cnt);
         if (b != cur_b) {
-            if (cur_b >= 0 && phase != 1) flush(cur_b);
-            for (int w = tid; w < ddn; w += nthr) {
-                dAacc[w] = 0;
-                facL[w] = A[(long long)b * ddn + w];
-            }
+            if (cur_b >= 0 && PHASE != 1) flush(cur_b);
+            for (int w = tid; w < ddn; w += nthr) facL[w] = A[(long long)b * ddn + w];
             cur_b = b;
         }
-        // the prefetched inputs into LDS
-        const int i0 = (phase == 1) ? 0 : (phase == 2) ? h - 1 : nf - 1;
-#pragma unroll
-        for (int k = 0; k < kAdjPer; ++k) {
-            const int w = tid + k * 256;
-            if (w < cnt) {
-                B(i0)[w] = v0[k];
-                if (phase != 1) B(i0 + 1)[w] = v1[k];
-            }
-        }
+        if (!staged) stage(ck);
         __syncthreads();
-        if (r + 1 < r1) load(r + 1);
+        int nb = b, nck_ = ck + 1;
+        if (nck_ == nck) {
+            nck_ = 0;
+            ++nb;
+        }
+        const bool more = r + 1 < r1;
+        if (more) load(nb, nck_);
 
-        if (phase == 1) {
-            // the slab chunk is C x n^nf; fast mode t sees outer C n^t, inner n^(nf-1-t)
-            int cur = 0;
+        int s0, Cx, cnt;
+        geom(ck, s0, Cx, cnt);
+        const T *res = nullptr;  // the buffer the item's result is stored from
+        if constexpr (PHASE == 1) {
+            // slab chunk Cx x n^nf; fast mode t sees outer Cx n^t, inner n^(nf-1-t)
+            int src = 0;
             for (int t = 0; t < nf; ++t) {
-                contract_mode_lds<T, Acc, true, SQ>(B(cur), B(1 - cur), C * ipow(n, t), n,
-                                                    ipow(n, nf - 1 - t), n, fac(h + t), n);
+                const int dst = 1 + (t & 1);
+                contract(B(src), B(dst), Cx * ipow(n, t), ipow(n, nf - 1 - t), h + t, TRUE_{});
                 __syncthreads();
-                cur = 1 - cur;
+                src = dst;
             }
-            T *V = W + (long long)b * 2 * E;
-            for (int w = tid; w < cnt; w += nthr) V[s0 * EF + w] = B(cur)[w];
-        } else if (phase == 2) {
-            // P_i in B(i) (B(h-1) holds X), G ping-pong in B(h), B(h+1); the
-            // column chunk is n^h x C, slow mode i sees outer n^i, inner n^(h-1-i) C
+            res = B(src);
+        } else if constexpr (PHASE == 2) {
+            // P_i in B(i) (B(h-1) holds X); column chunk n^h x Cx, slow mode i
+            // sees outer n^i, inner n^(h-1-i) Cx
             for (int i = h - 2; i >= 0; --i) {
-                contract_mode_lds<T, Acc, false, SQ>(B(i + 1), B(i), ipow(n, i + 1), n,
-                                                     ipow(n, h - 2 - i) * C, n, fac(i + 1), n);
+                contract(B(i + 1), B(i), ipow(n, i + 1), ipow(n, h - 2 - i) * Cx, i + 1, FALSE_{});
                 __syncthreads();
             }
-            if (nf > 0) {
-                // U chunk = A_0 x_0 P_0, through B(h+1) (the G chain starts in B(h))
-                contract_mode_lds<T, Acc, false, SQ>(B(0), B(h + 1), 1, n, ipow(n, h - 1) * C, n,
-                                                     fac(0), n);
+            // G_0 = V in B(h); G_1 -> B(h+2), G_(i+1) -> B(i-1) (P_(i-1) is
+            // dead by then); U = A_0 P_0 -> B(h+1)
+            int gb = h;
+            auto step = [&](auto ic) __attribute__((always_inline)) {
+                constexpr int i = decltype(ic)::value;
+                if (i >= h) return;
+                const int outer = ipow(n, i), inner = ipow(n, h - 1 - i) * Cx;
+                gram(B(gb), B(i), outer, inner, ic);
+                if (i == 0 && nf > 0) contract(B(0), B(h + 1), 1, inner, 0, FALSE_{});
+                const int dst = (i == 0) ? h + 2 : i - 1;
+                contract(B(gb), B(dst), outer, inner, i, TRUE_{});
                 __syncthreads();
-                T *U = W + (long long)b * 2 * E + E;
-                for (int w = tid; w < cnt; w += nthr) {
-                    const int is = w / C, c = w % C;
-                    U[(long long)is * EF + s0 + c] = B(h + 1)[w];
-                }
-                __syncthreads();
-            }
-            int g = h;
-            for (int i = 0; i < h; ++i) {
-                const int outer = ipow(n, i), inner = ipow(n, h - 1 - i) * C;
-                adj_gram_any<T, SQ>(B(g), B(i), outer, n, inner, dAacc + i * n * n);
-                const int gn = (g == h) ? h + 1 : h;
-                contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n, fac(i), n);
-                __syncthreads();
-                g = gn;
-            }
-            T *dx = dX + b * E;
-            for (int w = tid; w < cnt; w += nthr) {
-                const int is = w / C, c = w % C;
-                dx[(long long)is * EF + s0 + c] = B(g)[w];
-            }
+                gb = dst;
+            };
+            step(I0{});
+            step(std::integral_constant<int, 1>{});
+            step(std::integral_constant<int, 2>{});
+            step(std::integral_constant<int, 3>{});
+            step(std::integral_constant<int, 4>{});
+            step(std::integral_constant<int, 5>{});
+            res = B(gb);
         } else {
-            // P_t in B(t) (B(nf-1) holds U), G ping-pong in B(nf), B(nf+1); the
-            // slab chunk is C x n^nf, fast mode t sees outer C n^t, inner n^(nf-1-t)
+            // P_t in B(t) (B(nf-1) holds U); slab chunk Cx x n^nf, fast mode t
+            // sees outer Cx n^t, inner n^(nf-1-t); G_0 = dY in B(nf),
+            // G_1 -> B(nf+1), G_(t+1) -> B(t-1)
             for (int t = nf - 2; t >= 0; --t) {
-                contract_mode_lds<T, Acc, false, SQ>(B(t + 1), B(t), C * ipow(n, t + 1), n,
-                                                     ipow(n, nf - 2 - t), n, fac(h + t + 1), n);
+                contract(B(t + 1), B(t), Cx * ipow(n, t + 1), ipow(n, nf - 2 - t), h + t + 1,
+                         FALSE_{});
                 __syncthreads();
             }
-            int g = nf;
-            for (int t = 0; t < nf; ++t) {
-                const int outer = C * ipow(n, t), inner = ipow(n, nf - 1 - t);
-                adj_gram_any<T, SQ>(B(g), B(t), outer, n, inner, dAacc + (h + t) * n * n);
+            int gb = nf;
+            auto step = [&](auto tc) __attribute__((always_inline)) {
+                constexpr int t = decltype(tc)::value;
+                if (t >= nf) return;
+                const int outer = Cx * ipow(n, t), inner = ipow(n, nf - 1 - t);
+                gram(B(gb), B(t), outer, inner, tc);
                 if (t + 1 < nf) {
-                    const int gn = (g == nf) ? nf + 1 : nf;
-                    contract_mode_lds<T, Acc, true, SQ>(B(g), B(gn), outer, n, inner, n,
-                                                        fac(h + t), n);
-                    g = gn;
+                    const int dst = (t == 0) ? nf + 1 : t - 1;
+                    contract(B(gb), B(dst), outer, inner, h + t, TRUE_{});
+                    gb = dst;
                 }
                 __syncthreads();
+            };
+            step(I0{});
+            step(std::integral_constant<int, 1>{});
+            step(std::integral_constant<int, 2>{});
+            step(std::integral_constant<int, 3>{});
+            step(std::integral_constant<int, 4>{});
+        }
+
+        if (more) stage(nck_);
+        staged = more;
+
+        // stores of this item
+        const long long eb = (long long)b * E;
+        if constexpr (PHASE == 1) {
+            const auto gv = to_global(W + 2 * eb);
+            const SlabMap m = slab(s0, Cx);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    int w;
+                    const long long o = slab_map(m, q, w);
+                    gv[o] = res[w];
+                }
+            }
+        } else if constexpr (PHASE == 2) {
+            const auto gu = to_global(W + 2 * eb + E + ES * s0);
+            const auto gx = to_global(dX + eb);
+            const T *bu = B(h + 1);
+            const FastDiv dc = fast_div(Cx);
+#pragma unroll
+            for (int k = 0; k < kAdjPer; ++k) {
+                const int q = tid + k * 256;
+                if (q < cnt) {
+                    if (nf > 0) gu[q] = bu[q];
+                    const int is = quot(q, dc);
+                    gx[(long long)is * EF + s0 + (q - is * Cx)] = res[q];
+                }
             }
         }
-        __syncthreads();
+        (void)res;
+        b = nb;
+        ck = nck_;
     }
-    if (cur_b >= 0 && phase != 1) flush(cur_b);
+    if (cur_b >= 0 && PHASE != 1) flush(cur_b);
 }
 
 template <typename T>
@@ -323,12 +519,57 @@ static long long adjoint_lds_cap()
                                81920);
 }
 
-// Chunking for an LDS budget of `budget` elements (after the dA accumulator).
-static bool plan_adjoint(AdjShape &sh, long long budget)
+static long long adj_fixed_bytes(int d, int n, int elem)
+{
+    const long long ddn = (long long)d * n * n;
+    return (ddn * 8 + 15) / 16 * 16 + (ddn + 1) / 2 * 2 * elem;
+}
+
+// LDS buffers of each pass (chunk-sized): pass 1 a ping-pong pair plus the
+// input (two when one fast mode), pass 2 h + 3, pass 3 nf + 2.
+static int adj_nbuf(int phase, int h, int nf)
+{
+    if (phase == 1) return nf == 1 ? 2 : 3;
+    if (phase == 2) return h + 3;
+    return nf + 2;
+}
+
+// Estimated cost of one chunk of a pass, in CU cycles: the larger of the
+// instruction time (a contraction round: n^2 FMAs, 2n LDS accesses and index
+// arithmetic per fiber; a gram column: n FMAs and n+1 LDS reads; 4 cycles per
+// wave instruction) and the HBM time of its streams at about 8 bytes per CU
+// cycle.  A strided stream of runs of L bytes fetches about (L + 120) / L
+// times its size (the runs start anywhere in 128-byte lines).
+struct AdjCostIn {
+    int n, elem;
+    int contr;         // contractions per chunk
+    int grams;         // grams per chunk
+    long long fibers;  // fibers per contraction (= columns per gram)
+    int barriers;
+    double bytes;      // HBM bytes per chunk, run penalties included
+};
+static double adj_chunk_cost(const AdjCostIn &c)
+{
+    const int n = c.n;
+    const double rounds = (double)((c.fibers + 255) / 256);
+    const double parts = n > 0 ? (double)(256 / n) : 256.0;
+    const double gcols = std::ceil((double)c.fibers / parts);
+    const double instr = c.contr * (rounds * (n * n + 3.0 * n + 14) + n * n / 2.0 + 20) +
+                         c.grams * (gcols * (2.0 * n + 10) + 16) + c.barriers * 40.0;
+    const double compute = 4.0 * instr;
+    const double hbm = c.bytes / 8.0;
+    return std::max(compute, hbm) + 400.0;
+}
+static double run_penalty(double L) { return L >= 4096 ? 1.0 : (L + 120.0) / L; }
+
+// Chunking of an entry for the LDS budget `cap` (bytes): the split h and the
+// chunk sizes of minimal estimated cost.  False when no split fits.
+static bool plan_adjoint(AdjShape &sh, int elem, long long cap)
 {
     const int d = sh.d, n = sh.n;
+    const long long fixed = adj_fixed_bytes(d, n, elem);
     bool found = false;
-    long long best = 0;
+    double best = 0;
     AdjShape pick = sh;
     for (int h = 1; h <= d; ++h) {
         AdjShape s = sh;
@@ -337,23 +578,58 @@ static bool plan_adjoint(AdjShape &sh, long long budget)
         for (int k = 0; k < h; ++k) s.ES *= n;
         s.EF = s.E / s.ES;
         const int nf = d - h;
-        const int nb2 = h + 2, nb3 = nf + 2;
-        const int nbuf = std::max(nb2, nf > 0 ? std::max(2, nb3) : 0);
-        const long long per = std::min<long long>(budget / nbuf, kAdjPer * 256);
-        if (per < std::max(s.ES, nf > 0 ? s.EF : 1)) continue;
-        s.C1 = nf > 0 ? (int)std::min<long long>(s.ES, per / s.EF) : 1;
-        s.C2 = (int)std::min<long long>(s.EF, per / s.ES);
-        s.C3 = s.C1;
-        s.nbuf = nbuf;
-        s.buf = (int)std::max<long long>((long long)s.C2 * s.ES, nf > 0 ? (long long)s.C1 * s.EF : 0);
-        const long long iters = (nf > 0 ? 2 * ((s.ES + s.C1 - 1) / s.C1) : 0) +
-                                (s.EF + s.C2 - 1) / s.C2;
-        // favour column runs of at least 8 elements (pass 2 reads and writes
-        // rows of C2 consecutive elements: shorter rows waste HBM lines)
-        const long long cost = iters * (s.C2 < 8 && s.C2 < s.EF ? 4 : 1);
-        if (!found || cost < best) {
+        if (s.ES > kAdjChunk) break;
+        // pass 2: column chunks of width C2 (X and dX strided, V and U contiguous)
+        double c2best = -1;
+        for (long long C2 = 1; C2 <= std::min<long long>(s.EF, kAdjChunk / s.ES); ++C2) {
+            if (fixed + (long long)adj_nbuf(2, h, nf) * s.ES * C2 * elem > cap) break;
+            if (nf == 0 && C2 != 1) break;
+            const double runb = (double)C2 * elem;
+            const double vec = (double)s.ES * C2 * elem;
+            AdjCostIn c{n, elem, (h - 1) + h + (nf > 0 ? 1 : 0), h, s.ES * C2 / n, 2 * h - 1,
+                        vec * (run_penalty(runb) * 2 + (nf > 0 ? 2 : 0))};
+            const long long nck = (s.EF + C2 - 1) / C2;
+            const double cost = nck * adj_chunk_cost(c);
+            if (c2best < 0 || cost < c2best) {
+                c2best = cost;
+                s.C2 = (int)C2;
+                s.nck2 = (int)nck;
+            }
+        }
+        if (c2best < 0) continue;
+        double total = c2best;
+        s.C1 = s.C3 = 0;
+        if (nf > 0) {
+            if (s.EF > kAdjChunk) continue;
+            double c1best = -1, c3best = -1;
+            for (long long C = 1; C <= std::min<long long>(s.ES, kAdjChunk / s.EF); ++C) {
+                const double vec = (double)C * s.EF * elem;
+                const double runb = (double)C * s.C2 * elem;  // scratch runs per column chunk
+                const long long nck = (s.ES + C - 1) / C;
+                if (fixed + (long long)adj_nbuf(1, h, nf) * C * s.EF * elem <= cap) {
+                    AdjCostIn c{n, elem, nf, 0, C * s.EF / n, nf, vec * (1 + run_penalty(runb))};
+                    const double cost = nck * adj_chunk_cost(c);
+                    if (c1best < 0 || cost < c1best) {
+                        c1best = cost;
+                        s.C1 = (int)C;
+                    }
+                }
+                if (fixed + (long long)adj_nbuf(3, h, nf) * C * s.EF * elem <= cap) {
+                    AdjCostIn c{n, elem, 2 * nf - 2, nf, C * s.EF / n, 2 * nf - 1,
+                                vec * (1 + run_penalty(runb))};
+                    const double cost = nck * adj_chunk_cost(c);
+                    if (c3best < 0 || cost < c3best) {
+                        c3best = cost;
+                        s.C3 = (int)C;
+                    }
+                }
+            }
+            if (c1best < 0 || c3best < 0) continue;
+            total += c1best + c3best;
+        }
+        if (!found || total < best) {
             found = true;
-            best = cost;
+            best = total;
             pick = s;
         }
     }
@@ -361,17 +637,35 @@ static bool plan_adjoint(AdjShape &sh, long long budget)
     return found;
 }
 
-template <typename T>
-size_t adjoint_scratch_elems(int d, int n)
+static bool adjoint_shape(int d, int n, int elem, AdjShape &sh)
 {
     long long E = 1;
     for (int k = 0; k < d; ++k) E *= n;
-    AdjShape sh{d, n, 0, E, 0, 0, 0, 0, 0, 0, 0};
-    const long long fixed = ((long long)d * n * n * 8 + 15) / 16 * 16 +
-                            ((long long)d * n * n + 1) / 2 * 2 * (long long)sizeof(T);
-    const long long budget = (adjoint_lds_cap() - fixed) / (long long)sizeof(T);
-    if (budget <= 0 || !plan_adjoint(sh, budget)) return (size_t)-1;
-    return (sh.h < d) ? (size_t)(2 * E) : 0;
+    sh = AdjShape{d, n, 0, E, 0, 0, 0, 0, 0, 0};
+    return plan_adjoint(sh, elem, adjoint_lds_cap());
+}
+
+template <typename T>
+size_t adjoint_scratch_elems(int d, int n)
+{
+    AdjShape sh;
+    if (!adjoint_shape(d, n, (int)sizeof(T), sh)) return (size_t)-1;
+    return (sh.h < d) ? (size_t)(2 * sh.E) : 0;
+}
+
+template <typename T, int P>
+static const void *adjoint_kernel_for(int n)
+{
+    switch (n) {
+    case 2: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 2, P>);
+    case 3: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 3, P>);
+    case 4: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 4, P>);
+    case 5: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 5, P>);
+    case 6: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 6, P>);
+    case 7: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 7, P>);
+    case 8: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 8, P>);
+    default: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 0, P>);
+    }
 }
 
 template <typename T>
@@ -379,47 +673,37 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
                            int batch, hipStream_t stream)
 {
     if (batch == 0) return KRON_OK;
-    long long E = 1;
-    for (int k = 0; k < d; ++k) E *= n;
-    AdjShape sh{d, n, 0, E, 0, 0, 0, 0, 0, 0, 0};
-    const long long fixed = ((long long)d * n * n * 8 + 15) / 16 * 16 +
-                            ((long long)d * n * n + 1) / 2 * 2 * (long long)sizeof(T);
-    const long long lds_cap = adjoint_lds_cap();
-    const long long budget = (lds_cap - fixed) / (long long)sizeof(T);
-    if (budget <= 0 || !plan_adjoint(sh, budget)) return KRON_ENOTSUP;
+    AdjShape sh;
+    if (!adjoint_shape(d, n, (int)sizeof(T), sh)) return KRON_ENOTSUP;
     if (sh.h < d && !W) return KRON_EWORKSPACE;
-    const size_t lds = (size_t)fixed + (size_t)sh.nbuf * sh.buf * sizeof(T);
-    const void *kern;
-    switch (n) {
-    case 2: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 2>); break;
-    case 3: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 3>); break;
-    case 4: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 4>); break;
-    case 5: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 5>); break;
-    case 6: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 6>); break;
-    case 7: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 7>); break;
-    case 8: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 8>); break;
-    default: kern = reinterpret_cast<const void *>(kron_adjoint_kernel<T, T, 0>); break;
-    }
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
-        if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
-    }
-    const long long resident = persistent_blocks(kern, lds);
-    const long long cap = resident > 0 ? 2 * resident : 4096;
+    const int nf = d - sh.h;
     // dA is accumulated atomically by the work items: clear it first (a fill
     // kernel, so the sequence stays valid under stream capture)
     const long long nA = (long long)batch * d * n * n;
     hipLaunchKernelGGL(kron_fill_zero<T>, dim3((unsigned)std::min<long long>((nA + 255) / 256, 4096)),
                        dim3(256), 0, stream, dA, nA);
-    const int nf = d - sh.h;
+    const long long fixed = adj_fixed_bytes(d, n, (int)sizeof(T));
     for (int phase = 1; phase <= 3; ++phase) {
         if (phase != 2 && nf == 0) continue;
-        const long long nck = (phase == 2) ? (sh.EF + sh.C2 - 1) / sh.C2
-                                           : (sh.ES + sh.C1 - 1) / sh.C1;
+        const long long chunk = phase == 1   ? (long long)sh.C1 * sh.EF
+                                : phase == 2 ? sh.ES * sh.C2
+                                             : (long long)sh.C3 * sh.EF;
+        const size_t lds = (size_t)(fixed + adj_nbuf(phase, sh.h, nf) * chunk * (long long)sizeof(T));
+        const void *kern = phase == 1   ? adjoint_kernel_for<T, 1>(n)
+                           : phase == 2 ? adjoint_kernel_for<T, 2>(n)
+                                        : adjoint_kernel_for<T, 3>(n);
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds);
+            if (e != hipSuccess) { set_last_hip_error((int)e); return KRON_EHIP; }
+        }
+        const long long resident = persistent_blocks(kern, lds);
+        const long long cap = resident > 0 ? 2 * resident : 4096;
+        const long long nck = phase == 2 ? sh.nck2 : (sh.ES + (phase == 1 ? sh.C1 : sh.C3) - 1) /
+                                                         (phase == 1 ? sh.C1 : sh.C3);
         const long long grid = std::max(1LL, std::min((long long)batch * nck, cap));
-        void *args[] = {(void *)&A, (void *)&X, (void *)&dY, (void *)&dX, (void *)&dA, (void *)&W,
-                        (void *)&sh, (void *)&batch, (void *)&phase};
+        void *args[] = {(void *)&A,  (void *)&X, (void *)&dY, (void *)&dX,
+                        (void *)&dA, (void *)&W, (void *)&sh, (void *)&batch};
         hipError_t le = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, lds, stream);
         if (le != hipSuccess) { set_last_hip_error((int)le); return KRON_EHIP; }
     }
