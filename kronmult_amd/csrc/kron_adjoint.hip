@@ -95,6 +95,20 @@ __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__res
     }
 }
 
+// Offset (o * n + j) * inner + ii of element j of fiber x = o * inner + ii,
+// split so the element offsets j * inner stay uniform adds: base = o * n *
+// inner + ii with 24-bit multiplies (full rate; chunk offsets are far below
+// 2^24), the steps hidden from the optimiser, which would otherwise fold them
+// into 32- or 64-bit multiplies (quarter rate), one per element.
+__device__ __forceinline__ int fiber_base(int x, int o, int inner, int n_inner)
+{
+    int ii = x - (int)__umul24((unsigned)o, (unsigned)inner);
+    asm volatile("" : "+v"(ii));
+    int base = (int)__umul24((unsigned)o, (unsigned)n_inner) + ii;
+    asm volatile("" : "+v"(base));
+    return base;
+}
+
 // Row p = t % N of the gram G P^T over the columns (o, ii) of two
 // outer x N x inner chunks, added into the thread's accumulators: thread t
 // takes the columns part, part + PARTS, ... (part = t / N) and reads one G
@@ -109,13 +123,15 @@ __device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *_
     const int p = t % N, part = t / N;
     const int cols = outer * inner;
     const FastDiv di = fast_div(inner);
+    const T *Gp = G + p * inner;
 #pragma unroll 2
     for (int c = part; c < cols; c += PARTS) {
-        const int o = quot(c, di), ii = c - o * inner;
-        const int base = (o * N) * inner + ii;
-        const double g = double(G[base + p * inner]);
+        const int o = quot(c, di);
+        const int base = fiber_base(c, o, inner, N * inner);
+        const T *Pb = P + base;
+        const double g = double(Gp[base]);
 #pragma unroll
-        for (int r = 0; r < N; ++r) acc[r] = __builtin_fma(g, double(P[base + r * inner]), acc[r]);
+        for (int r = 0; r < N; ++r) acc[r] = __builtin_fma(g, double(Pb[r * inner]), acc[r]);
     }
 }
 
@@ -136,17 +152,19 @@ __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__res
         for (int j = 0; j < N; ++j) a[i][j] = TRANS ? Al[j + i * N] : Al[i + j * N];
     const FastDiv di = fast_div(inner);
     for (int f = threadIdx.x; f < fibers; f += 256) {
-        const int o = quot(f, di), ii = f - o * inner;
-        const int base = (o * N) * inner + ii;
+        const int o = quot(f, di);
+        const int base = fiber_base(f, o, inner, N * inner);
+        const T *s = src + base;
+        T *dd = dst + base;
         T in[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) in[j] = src[base + j * inner];
+        for (int j = 0; j < N; ++j) in[j] = s[j * inner];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             T acc = a[i][0] * in[0];
 #pragma unroll
             for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
-            dst[base + i * inner] = acc;
+            dd[i * inner] = acc;
         }
     }
 }
@@ -161,18 +179,21 @@ constexpr int kAdjChunk = kAdjPer * 256;
 // scratch offset; w is the element's index in the row-major slab (rows x EF).
 struct SlabMap {
     FastDiv blk, full, last;  // rows * C2, C2, width of the last chunk
-    int EF, C2, nck2, s0;
-    long long ES;
+    int EF, C2, nck2, s0, ES;
 };
-__device__ __forceinline__ long long slab_map(const SlabMap &m, int q, int &w)
+// Offsets within an entry are below n^d <= 2^22 (the planner's chunk limits),
+// so 24-bit multiplies suffice.
+__device__ __forceinline__ unsigned mul24(int a, int b) { return __umul24((unsigned)a, (unsigned)b); }
+__device__ __forceinline__ int slab_map(const SlabMap &m, int q, int &w)
 {
     const int c = quot(q, m.blk);
-    const int rem = q - c * m.blk.d;
+    const int rem = q - (int)mul24(c, m.blk.d);
     const bool last = c == m.nck2 - 1;
     const int Cc = last ? m.last.d : m.C2;
     const int r = quot(rem, last ? m.last : m.full);
-    w = r * m.EF + c * m.C2 + (rem - r * Cc);
-    return m.ES * (long long)(c * m.C2) + (long long)m.s0 * Cc + rem;
+    const int cc0 = (int)mul24(c, m.C2);
+    w = (int)mul24(r, m.EF) + cc0 + (rem - (int)mul24(r, Cc));
+    return (int)mul24(m.ES, cc0) + (int)mul24(m.s0, Cc) + rem;
 }
 
 // One pass of the adjoint (PHASE 1, 2 or 3) over work items (entry, chunk).
@@ -230,7 +251,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
         m.C2 = sh.C2;
         m.nck2 = sh.nck2;
         m.s0 = s0;
-        m.ES = ES;
+        m.ES = (int)ES;
         return m;
     };
 
@@ -244,10 +265,11 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
         const long long eb = (long long)b * E;
         const auto gdy = to_global(dY + eb);
         if constexpr (PHASE == 1) {
+            const int sb = (int)mul24(s0, (int)EF);
 #pragma unroll
             for (int k = 0; k < kAdjPer; ++k) {
                 const int q = tid + k * 256;
-                if (q < cnt) v0[k] = gdy[(long long)s0 * EF + q];
+                if (q < cnt) v0[k] = gdy[sb + q];
             }
         } else if constexpr (PHASE == 2) {
             const auto gx = to_global(X + eb);
@@ -258,7 +280,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
                 const int q = tid + k * 256;
                 if (q < cnt) {
                     const int is = quot(q, dc);
-                    const long long g = (long long)is * EF + s0 + (q - is * Cx);
+                    const int g = (int)mul24(is, (int)EF) + s0 + (q - (int)mul24(is, Cx));
                     v0[k] = gx[g];
                     v1[k] = nf > 0 ? gv[q] : gdy[g];
                 }
@@ -266,13 +288,14 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
         } else {
             const auto gu = to_global(W + 2 * eb + E);
             const SlabMap m = slab(s0, Cx);
+            const int sb = (int)mul24(s0, (int)EF);
 #pragma unroll
             for (int k = 0; k < kAdjPer; ++k) {
                 const int q = tid + k * 256;
                 if (q < cnt) {
                     int w;
                     v0[k] = gu[slab_map(m, q, w)];
-                    v1[k] = gdy[(long long)s0 * EF + q];
+                    v1[k] = gdy[sb + q];
                 }
             }
         }
@@ -478,7 +501,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
                 const int q = tid + k * 256;
                 if (q < cnt) {
                     int w;
-                    const long long o = slab_map(m, q, w);
+                    const int o = slab_map(m, q, w);
                     gv[o] = res[w];
                 }
             }
@@ -493,7 +516,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
                 if (q < cnt) {
                     if (nf > 0) gu[q] = bu[q];
                     const int is = quot(q, dc);
-                    gx[(long long)is * EF + s0 + (q - is * Cx)] = res[q];
+                    gx[(int)mul24(is, (int)EF) + s0 + (q - (int)mul24(is, Cx))] = res[q];
                 }
             }
         }
