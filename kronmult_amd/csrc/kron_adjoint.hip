@@ -47,17 +47,6 @@ __device__ __forceinline__ int ipow(int n, int k)
     return p;
 }
 
-// Quotient of small non-negative integers through the FP32 reciprocal:
-// int((x + 1/2) * rcp(d)) == x / d for 0 <= x < 2^21.  (x + 1/2) / d lies at
-// least 1/(2d) from every integer, and the rounding of rcp (1 ulp) and of the
-// product moves it by less than (x + 1/2) * 2^-22 / d < 1/(2d).
-struct FastDiv {
-    float r;
-    int d;
-};
-__device__ __forceinline__ FastDiv fast_div(int d) { return {__builtin_amdgcn_rcpf(float(d)), d}; }
-__device__ __forceinline__ int quot(int x, FastDiv f) { return int((float(x) + 0.5f) * f.r); }
-
 // dA(p, r) += sum_{o, ii} G[o, p, ii] * P[o, r, ii] over tensors viewed as
 // outer x n x inner (dA in LDS, column-major n x n, double accumulation), for
 // a runtime n.  Thread t takes the pair t % npair and every (t / npair)-th
@@ -95,20 +84,6 @@ __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__res
     }
 }
 
-// Offset (o * n + j) * inner + ii of element j of fiber x = o * inner + ii,
-// split so the element offsets j * inner stay uniform adds: base = o * n *
-// inner + ii with 24-bit multiplies (full rate; chunk offsets are far below
-// 2^24), the steps hidden from the optimiser, which would otherwise fold them
-// into 32- or 64-bit multiplies (quarter rate), one per element.
-__device__ __forceinline__ int fiber_base(int x, int o, int inner, int n_inner)
-{
-    int ii = x - (int)__umul24((unsigned)o, (unsigned)inner);
-    asm volatile("" : "+v"(ii));
-    int base = (int)__umul24((unsigned)o, (unsigned)n_inner) + ii;
-    asm volatile("" : "+v"(base));
-    return base;
-}
-
 // Row p = t % N of the gram G P^T over the columns (o, ii) of two
 // outer x N x inner chunks, added into the thread's accumulators: thread t
 // takes the columns part, part + PARTS, ... (part = t / N) and reads one G
@@ -126,8 +101,9 @@ __device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *_
     const T *Gp = G + p * inner;
 #pragma unroll 2
     for (int c = part; c < cols; c += PARTS) {
-        const int o = quot(c, di);
-        const int base = fiber_base(c, o, inner, N * inner);
+        int ii;
+        const int o = split_fiber(c, di, ii);
+        const int base = fiber_base(o, ii, N * inner);
         const T *Pb = P + base;
         const double g = double(Gp[base]);
 #pragma unroll
@@ -152,8 +128,9 @@ __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__res
         for (int j = 0; j < N; ++j) a[i][j] = TRANS ? Al[j + i * N] : Al[i + j * N];
     const FastDiv di = fast_div(inner);
     for (int f = threadIdx.x; f < fibers; f += 256) {
-        const int o = quot(f, di);
-        const int base = fiber_base(f, o, inner, N * inner);
+        int ii;
+        const int o = split_fiber(f, di, ii);
+        const int base = fiber_base(o, ii, N * inner);
         const T *s = src + base;
         T *dd = dst + base;
         T in[N];
@@ -182,8 +159,7 @@ struct SlabMap {
     int EF, C2, nck2, s0, ES;
 };
 // Offsets within an entry are below n^d <= 2^22 (the planner's chunk limits),
-// so 24-bit multiplies suffice.
-__device__ __forceinline__ unsigned mul24(int a, int b) { return __umul24((unsigned)a, (unsigned)b); }
+// so 24-bit multiplies (mul24) suffice.
 __device__ __forceinline__ int slab_map(const SlabMap &m, int q, int &w)
 {
     const int c = quot(q, m.blk);

@@ -178,6 +178,39 @@ __device__ __forceinline__ void y_accumulate(P y, long long idx, Acc v)
     }
 }
 
+// Quotient of small non-negative integers through the FP32 reciprocal:
+// int((x + 1/2) * rcp(d)) == x / d for 0 <= x < 2^21.  (x + 1/2) / d lies at
+// least 1/(2d) from every integer, and the rounding of rcp (1 ulp) and of the
+// product moves it by less than (x + 1/2) * 2^-22 / d < 1/(2d).
+struct FastDiv {
+    float r;
+    int d;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return {__builtin_amdgcn_rcpf(float(d)), d}; }
+__device__ __forceinline__ int quot(int x, FastDiv f) { return int((float(x) + 0.5f) * f.r); }
+// Full-rate 24-bit multiply (LDS and in-vector offsets are far below 2^24).
+__device__ __forceinline__ unsigned mul24(int a, int b) { return __umul24((unsigned)a, (unsigned)b); }
+
+// Fiber x = o * inner + ii of a tensor viewed as outer x n x inner: o, and ii
+// hidden from the optimiser (see fiber_base).
+__device__ __forceinline__ int split_fiber(int x, FastDiv inner, int &ii)
+{
+    const int o = quot(x, inner);
+    ii = x - (int)mul24(o, inner.d);
+    asm volatile("" : "+v"(ii));
+    return o;
+}
+// Offset o * n * inner + ii of the fiber's first element; element j is then
+// j * inner further (a uniform add).  Hidden from the optimiser, which would
+// otherwise fold the sum back into (o * n + j) * inner + ii: one quarter-rate
+// 32- or 64-bit multiply per element.
+__device__ __forceinline__ int fiber_base(int o, int ii, int n_inner)
+{
+    int base = (int)mul24(o, n_inner) + ii;
+    asm volatile("" : "+v"(base));
+    return base;
+}
+
 // dst = op(A) x_q src for one mode of an LDS tensor viewed as
 // outer x n x inner (input) and outer x m x inner (output), column-major with
 // the inner index fastest; A is m x n (n x m stored when TRANS) in LDS with
@@ -189,12 +222,14 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
                                                 int outer, int n, int inner, int m,
                                                 const T *__restrict__ Al, int ldA)
 {
+    const FastDiv di = fast_div(inner);
     if (n <= NMAX) {
         const int fibers = outer * inner;
         for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
-            const int ii = f % inner, o = f / inner;
-            const T *s = src + (o * n) * inner + ii;
-            T *dd = dst + (o * m) * inner + ii;
+            int ii;
+            const int o = split_fiber(f, di, ii);
+            const T *s = src + fiber_base(o, ii, n * inner);
+            T *dd = dst + fiber_base(o, ii, m * inner);
             Acc in[NMAX];
 #pragma unroll
             for (int j = 0; j < NMAX; ++j) in[j] = (j < n) ? Acc(s[j * inner]) : Acc(0);
@@ -210,12 +245,13 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
         }
     } else {
         const int total = outer * m * inner;
+        const FastDiv dm = fast_div(m);
         for (int w = threadIdx.x; w < total; w += blockDim.x) {
-            const int ii = w % inner;
-            const int rest = w / inner;
-            const int i = rest % m;
-            const int o = rest / m;
-            const T *s = src + (o * n) * inner + ii;
+            int ii;
+            const int rest = split_fiber(w, di, ii);
+            const int o = quot(rest, dm);
+            const int i = rest - (int)mul24(o, m);
+            const T *s = src + fiber_base(o, ii, n * inner);
             Acc acc = 0;
             for (int j = 0; j < n; ++j)
                 acc = __builtin_fma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
@@ -237,6 +273,8 @@ __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *
         contract_fibers<T, Acc, TRANS, 8>(src, dst, outer, n, inner, m, Al, ldA);
     } else {
         constexpr int N = SQ;
+        const int fibers = outer * inner;
+        if ((int)threadIdx.x >= fibers) return;
         // the factor is the same for the whole workgroup; it is held in VGPRs
         // (a scalar copy of N*N doubles overflows the SGPR file above N = 3)
         Acc a[N][N];
@@ -245,19 +283,22 @@ __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *
 #pragma unroll
             for (int j = 0; j < N; ++j)
                 a[i][j] = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
-        const int fibers = outer * inner;
+        const FastDiv di = fast_div(inner);
         for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
-            const int ii = f % inner, o = f / inner;
-            const int base = (o * N) * inner + ii;
+            int ii;
+            const int o = split_fiber(f, di, ii);
+            const int base = fiber_base(o, ii, N * inner);
+            const T *s = src + base;
+            T *dd = dst + base;
             Acc in[N];
 #pragma unroll
-            for (int j = 0; j < N; ++j) in[j] = Acc(src[base + j * inner]);
+            for (int j = 0; j < N; ++j) in[j] = Acc(s[j * inner]);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 Acc acc = a[i][0] * in[0];
 #pragma unroll
                 for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
-                dst[base + i * inner] = T(acc);
+                dd[i * inner] = T(acc);
             }
         }
     }

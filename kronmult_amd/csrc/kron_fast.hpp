@@ -448,24 +448,50 @@ constexpr int fast_max_n(int D)
                             : (D == 6 ? 5 : 8);
 }
 
+}  // namespace kron
+
+// (after load_factor, which the plane kernel uses)
+#include "kron_plane.hpp"
+
+namespace kron {
+
+// The d = 6, n = 4 entry runs the two-modes-per-pass kernel (kron_plane.hpp).
+template <int D, int N>
+constexpr bool fast_uses_plane()
+{
+    return D == 6 && N == 4;
+}
+
+template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS,
+          bool GROUPED>
+constexpr auto fast_kernel_for()
+{
+    if constexpr (fast_uses_plane<D, N>())
+        return kron_plane_kernel<T, Acc, LAY, SHARED, TRANS, GROUPED>;
+    else
+        return kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, GROUPED>;
+}
+
 template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS>
 kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipStream_t stream,
                             const GroupOrder *ord)
 {
     using Geo = FastGeom<D, N>;
-    const size_t lds = (D == 1) ? 0 : Geo::lds_bytes(sizeof(T));
+    const size_t lds = (D == 1)                    ? 0
+                       : fast_uses_plane<D, N>() ? (size_t)kPlaneLds * sizeof(T)
+                                                 : Geo::lds_bytes(sizeof(T));
     if ((long long)lds > (long long)max_lds_bytes()) return KRON_ENOTSUP;
     constexpr bool CAN_GROUP = SHARED && fast_groupable<T, D, N>();
     const bool grouped = CAN_GROUP && ord != nullptr;
     const void *kern;
     if constexpr (CAN_GROUP) {
         kern = grouped ? reinterpret_cast<const void *>(
-                             kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, true>)
+                             fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, true>())
                        : reinterpret_cast<const void *>(
-                             kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>);
+                             fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>());
     } else {
         kern = reinterpret_cast<const void *>(
-            kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>);
+            fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>());
     }
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -491,15 +517,15 @@ kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipS
     if (nblocks > 0x7fffffffLL) return KRON_EINVAL;
     if constexpr (CAN_GROUP) {
         if (grouped)
-            hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, true>),
+            hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, true>()),
                                dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
                                o);
         else
-            hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>),
+            hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>()),
                                dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
                                o);
     } else {
-        hipLaunchKernelGGL((kron_fast_kernel<T, Acc, D, N, LAY, SHARED, TRANS, false>),
+        hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>()),
                            dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch, o);
     }
     hipError_t e = hipGetLastError();

@@ -87,6 +87,11 @@ struct GenericPasses {
     long long nck;
     T *buf0, *buf1, *facs;
 
+    // LDS buffer b (0 or 1) of the ping-pong pair
+    __device__ __forceinline__ T *buf(int b) const { return b ? buf1 : buf0; }
+    // modes contracted by this pass
+    __device__ __forceinline__ int modes() const { return phase == 1 ? sh.h : sh.d - sh.h; }
+
     __device__ __forceinline__ void decode(long long r, int &b, long long &c0, int &C) const
     {
         b = b0 + (int)(r / nck);
@@ -115,16 +120,25 @@ struct GenericPasses {
         decode(r, b, c0, C);
         const auto src = to_global(src_of(b));
         const int cnt = count(C);
+        if (phase == 1) {
+            // rows of C consecutive low indices, nlo_in apart
+            const auto s1 = src + c0;
+            const FastDiv dc = fast_div(C);
 #pragma unroll
-        for (int k = 0; k < kStagePer; ++k) {
-            const int w = threadIdx.x + k * 256;
-            if (w < cnt) {
-                if (phase == 1) {
-                    const int hi = w / C, c = w % C;
-                    st.v[k] = src[(long long)hi * sh.nlo_in + c0 + c];
-                } else {
-                    st.v[k] = src[c0 * sh.nlo_in + w];
+            for (int k = 0; k < kStagePer; ++k) {
+                const int w = threadIdx.x + k * 256;
+                if (w < cnt) {
+                    int c;
+                    const int hi = split_fiber(w, dc, c);
+                    st.v[k] = s1[(long long)hi * sh.nlo_in + c];
                 }
+            }
+        } else {
+            const auto s2 = src + c0 * sh.nlo_in;
+#pragma unroll
+            for (int k = 0; k < kStagePer; ++k) {
+                const int w = threadIdx.x + k * 256;
+                if (w < cnt) st.v[k] = s2[w];
             }
         }
         if constexpr (!SHARED) {
@@ -140,12 +154,12 @@ struct GenericPasses {
             }
         }
     }
-    __device__ __forceinline__ void to_lds(const RegStage<T> &st, int cnt) const
+    __device__ __forceinline__ void to_lds(const RegStage<T> &st, int cnt, T *dst) const
     {
 #pragma unroll
         for (int k = 0; k < kStagePer; ++k) {
             const int w = threadIdx.x + k * 256;
-            if (w < cnt) buf0[w] = st.v[k];
+            if (w < cnt) dst[w] = st.v[k];
         }
     }
     __device__ __forceinline__ void stage_factors(int b) const
@@ -156,12 +170,12 @@ struct GenericPasses {
                          sh.ld[k]);
         }
     }
-    // contract the item's modes in LDS (input in buf0) and store the result
-    __device__ __forceinline__ void compute_store(int b, long long c0, int C,
-                                                  const RegStage<T> &ycur) const
+    // contract the item's modes in LDS, input in buf(in); the result ends in
+    // buf(in ^ (modes() & 1))
+    __device__ __forceinline__ void compute(int C, int in) const
     {
-        const int tid = threadIdx.x, nthr = blockDim.x, h = sh.h, d = sh.d;
-        T *src = buf0, *dst = buf1;
+        const int h = sh.h, d = sh.d;
+        T *src = buf(in), *dst = buf(in ^ 1);
         if (phase == 1) {
             // the chunk is n_0 x .. x n_{h-1} x C; mode q sees outer = m_0..m_{q-1}
             // (already contracted) and inner = n_{q+1}..n_{h-1} x C
@@ -176,12 +190,6 @@ struct GenericPasses {
                 if (q + 1 < h) inner /= sh.n[q + 1];
                 __syncthreads();
                 T *t = src; src = dst; dst = t;
-            }
-            T *S = scr.get(b - b0, b);
-            const int nout = (int)sh.nhi_out * C;
-            for (int w = tid; w < nout; w += nthr) {
-                const int hi = w / C, c = w % C;
-                S[(long long)hi * sh.nlo_in + c0 + c] = src[w];
             }
         } else {
             // the chunk is C x n_h x .. x n_{d-1}; mode k sees outer = C x m_h..m_{k-1}
@@ -198,6 +206,28 @@ struct GenericPasses {
                 __syncthreads();
                 T *t = src; src = dst; dst = t;
             }
+        }
+    }
+    // store the item's result (in res): the scratch chunk (pass 1) or Y
+    __device__ __forceinline__ void store(int b, long long c0, int C, const T *res,
+                                          const RegStage<T> &ycur) const
+    {
+        const int tid = threadIdx.x, nthr = blockDim.x;
+        const T *src = res;
+        if (phase == 1) {
+            const auto S = to_global(scr.get(b - b0, b) + c0);
+            const int nout = (int)sh.nhi_out * C;
+            const FastDiv dc = fast_div(C);
+#pragma unroll
+            for (int k = 0; k < kStagePer; ++k) {
+                const int w = tid + k * 256;
+                if (w < nout) {
+                    int c;
+                    const int hi = split_fiber(w, dc, c);
+                    S[(long long)hi * sh.nlo_in + c] = src[w];
+                }
+            }
+        } else {
             const auto y = to_global(ent.y(b));
             const int nout = C * (int)sh.nlo_out;
             const long long ybase = c0 * sh.nlo_out;
@@ -239,6 +269,9 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
     RegStage<T> st;
     if (r0 < r1) P.load(st, r0);
     int cur_b = -1;
+    int in = 0;  // LDS buffer holding the current item's input
+    bool staged = false;
+    const int flip = P.modes() & 1;  // the result lands in buf(in ^ flip)
     for (long long r = r0; r < r1; ++r) {
         int b, C;
         long long c0;
@@ -247,16 +280,31 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
             P.stage_factors(b);
             cur_b = b;
         }
-        P.to_lds(st, P.count(C));
+        if (!staged) P.to_lds(st, P.count(C), P.buf(in));
         RegStage<T> ycur;
         if constexpr (!SHARED) {
 #pragma unroll
             for (int k = 0; k < kStagePer; ++k) ycur.y[k] = st.y[k];
         }
         __syncthreads();
-        if (r + 1 < r1) P.load(st, r + 1);
-        P.compute_store(b, c0, C, ycur);
-        __syncthreads();
+        const bool more = r + 1 < r1;
+        if (more) P.load(st, r + 1);
+        P.compute(C, in);
+        // The next item's input goes into the buffer the result is not in,
+        // before this item's stores are issued: waiting for its loads then
+        // does not also wait for those stores (the memory counter is in order).
+        const int res = in ^ flip, next = res ^ 1;
+        if (more) {
+            int nb, nC;
+            long long nc0;
+            P.decode(r + 1, nb, nc0, nC);
+            P.to_lds(st, P.count(nC), P.buf(next));
+        }
+        staged = more;
+        P.store(b, c0, C, P.buf(res), ycur);
+        in = next;
+        // (the barrier at the top of the next item orders these LDS reads
+        // before that item's contractions overwrite buf(res))
     }
 }
 
