@@ -179,7 +179,7 @@ __device__ __forceinline__ int slab_map(const SlabMap &m, int q, int &w)
 // data also waits for stores of the same item.  Input buffers are never the
 // buffers results are stored from.  N > 0 is the compile-time factor size
 // (thread gram rows in registers for N <= 6); N == 0 handles any n.
-template <typename T, int N, int PHASE>
+template <typename T, int N, int PHASE, int NFC = 0>
 __global__ void __launch_bounds__(256)
 kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *__restrict__ dY,
                     T *__restrict__ dX, T *__restrict__ dA, T *__restrict__ W, AdjShape sh,
@@ -189,7 +189,9 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
     constexpr bool KEEP = N > 0 && N <= 6;  // gram rows kept across an entry's chunks
     constexpr int GA = KEEP ? (PHASE == 2 ? kMaxDim : kMaxDim - 1) : 1;
     constexpr int NR = N > 0 ? N : 1;
-    const int d = sh.d, n = N > 0 ? N : sh.n, h = sh.h, nf = d - h;
+    // NFC > 0: the number of fast modes (passes 1 and 3) as a compile-time
+    // constant, so every stride n^k of those passes is one
+    const int d = sh.d, n = N > 0 ? N : sh.n, h = sh.h, nf = NFC > 0 ? NFC : d - h;
     const int nn = n * n, ddn = d * nn;
     double *dAacc = reinterpret_cast<double *>(kron_smem);  // d*n*n
     T *facL = reinterpret_cast<T *>(kron_smem + ((size_t)ddn * sizeof(double) + 15) / 16 * 16);
@@ -652,17 +654,32 @@ size_t adjoint_scratch_elems(int d, int n)
     return (sh.h < d) ? (size_t)(2 * sh.E) : 0;
 }
 
+template <typename T, int N, int P>
+static const void *adjoint_kernel_nf(int nf)
+{
+    // compile-time fast-mode counts for the common split (passes 1 and 3)
+    if constexpr (P != 2 && N >= 4 && N <= 6) {
+        switch (nf) {
+        case 2: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 2>);
+        case 3: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 3>);
+        case 4: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 4>);
+        default: break;
+        }
+    }
+    return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P>);
+}
+
 template <typename T, int P>
-static const void *adjoint_kernel_for(int n)
+static const void *adjoint_kernel_for(int n, int nf)
 {
     switch (n) {
-    case 2: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 2, P>);
-    case 3: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 3, P>);
-    case 4: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 4, P>);
-    case 5: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 5, P>);
-    case 6: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 6, P>);
-    case 7: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 7, P>);
-    case 8: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 8, P>);
+    case 2: return adjoint_kernel_nf<T, 2, P>(nf);
+    case 3: return adjoint_kernel_nf<T, 3, P>(nf);
+    case 4: return adjoint_kernel_nf<T, 4, P>(nf);
+    case 5: return adjoint_kernel_nf<T, 5, P>(nf);
+    case 6: return adjoint_kernel_nf<T, 6, P>(nf);
+    case 7: return adjoint_kernel_nf<T, 7, P>(nf);
+    case 8: return adjoint_kernel_nf<T, 8, P>(nf);
     default: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 0, P>);
     }
 }
@@ -688,9 +705,9 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
                                 : phase == 2 ? sh.ES * sh.C2
                                              : (long long)sh.C3 * sh.EF;
         const size_t lds = (size_t)(fixed + adj_nbuf(phase, sh.h, nf) * chunk * (long long)sizeof(T));
-        const void *kern = phase == 1   ? adjoint_kernel_for<T, 1>(n)
-                           : phase == 2 ? adjoint_kernel_for<T, 2>(n)
-                                        : adjoint_kernel_for<T, 3>(n);
+        const void *kern = phase == 1   ? adjoint_kernel_for<T, 1>(n, nf)
+                           : phase == 2 ? adjoint_kernel_for<T, 2>(n, nf)
+                                        : adjoint_kernel_for<T, 3>(n, nf);
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)lds);
