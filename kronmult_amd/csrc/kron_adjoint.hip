@@ -112,12 +112,13 @@ __device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *_
 }
 
 // One mode contraction of a chunk held in LDS as outer x N x inner:
-// dst(o, i, ii) = sum_j F(i, j) src(o, j, ii), F = A_k or A_k^T (column-major
-// in LDS).  Thread t contracts the fibers t, t + 256, ...; the factor is read
-// into VGPRs (a broadcast LDS read per value).
-template <typename T, int N, bool TRANS>
+// dst(o, i, ii) = sum_j F(i, j) src(o, j, ii), F = A_k or A_k^T (column-major,
+// read through Al: the factor in global memory through the constant address
+// space -- scalar loads into SGPRs -- or its LDS copy).  Thread t contracts
+// the fibers t, t + 256, ...
+template <typename T, int N, bool TRANS, typename FacP>
 __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__restrict__ dst,
-                                             int outer, int inner, const T *__restrict__ Al)
+                                             int outer, int inner, FacP Al)
 {
     const int fibers = outer * inner;
     if ((int)threadIdx.x >= fibers) return;
@@ -340,11 +341,16 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             }
         }
     };
+    int cur_b = -1;  // the entry whose factors are staged
     auto contract = [&](const T *src, T *dst, int outer, int inner, int k, auto tr)
                         __attribute__((always_inline)) {
         constexpr bool TR = decltype(tr)::value;
         if constexpr (N == 0)
             contract_fibers<T, T, TR, 8>(src, dst, outer, n, inner, n, fac(k), n);
+        else if constexpr (N * N * sizeof(T) <= 288)
+            // the factor through scalar loads (SGPRs): no LDS reads for it
+            adj_contract<T, N, TR>(src, dst, outer, inner,
+                                   to_const(A + (long long)cur_b * ddn + k * nn));
         else
             adj_contract<T, N, TR>(src, dst, outer, inner, fac(k));
     };
@@ -378,7 +384,6 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
     for (int w = tid; w < ddn; w += nthr) dAacc[w] = 0;
     int b = (int)(r0 / nck), ck = (int)(r0 % nck);
     if (r0 < r1) load(b, ck);
-    int cur_b = -1;
     bool staged = false;
     for (long long r = r0; r < r1; ++r) {
         if (b != cur_b) {
