@@ -264,6 +264,38 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
 // Square variant with compile-time N: the factor is read once from LDS into
 // registers (N*N values) and each thread contracts whole fibers, N loads and
 // N*N FMAs each.  SQ == 0 selects contract_fibers (runtime extents).
+// dst = A x_q src for one mode of an LDS tensor viewed as outer x N x inner,
+// with the N x N factor already in registers (a[i][j] = A(i, j)): each
+// thread contracts whole fibers, N loads, N*N FMAs and N stores each.
+template <typename T, typename Acc, int N>
+__device__ __forceinline__ void contract_fibers_reg(const T *__restrict__ src,
+                                                    T *__restrict__ dst, int outer, int inner,
+                                                    const Acc (&a)[N][N])
+{
+    const int fibers = outer * inner;
+    const FastDiv di = fast_div(inner);
+    for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
+        int ii;
+        const int o = split_fiber(f, di, ii);
+        const int base = fiber_base(o, ii, N * inner);
+        const T *s = src + base;
+        T *dd = dst + base;
+        Acc in[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) in[j] = Acc(s[j * inner]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            Acc acc = a[i][0] * in[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
+            dd[i * inner] = T(acc);
+        }
+    }
+}
+
+// Square variant with compile-time N: the factor is read once from LDS into
+// registers (N*N values), then contract_fibers_reg.  SQ == 0 selects
+// contract_fibers (runtime extents).
 template <typename T, typename Acc, bool TRANS, int SQ>
 __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *__restrict__ dst,
                                                   int outer, int n, int inner, int m,
@@ -273,34 +305,14 @@ __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *
         contract_fibers<T, Acc, TRANS, 8>(src, dst, outer, n, inner, m, Al, ldA);
     } else {
         constexpr int N = SQ;
-        const int fibers = outer * inner;
-        if ((int)threadIdx.x >= fibers) return;
-        // the factor is the same for the whole workgroup; it is held in VGPRs
-        // (a scalar copy of N*N doubles overflows the SGPR file above N = 3)
+        if ((int)threadIdx.x >= outer * inner) return;
         Acc a[N][N];
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll
             for (int j = 0; j < N; ++j)
                 a[i][j] = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
-        const FastDiv di = fast_div(inner);
-        for (int f = threadIdx.x; f < fibers; f += blockDim.x) {
-            int ii;
-            const int o = split_fiber(f, di, ii);
-            const int base = fiber_base(o, ii, N * inner);
-            const T *s = src + base;
-            T *dd = dst + base;
-            Acc in[N];
-#pragma unroll
-            for (int j = 0; j < N; ++j) in[j] = Acc(s[j * inner]);
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                Acc acc = a[i][0] * in[0];
-#pragma unroll
-                for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
-                dd[i * inner] = T(acc);
-            }
-        }
+        contract_fibers_reg<T, Acc, N>(src, dst, outer, inner, a);
     }
 }
 

@@ -170,9 +170,26 @@ struct GenericPasses {
                          sh.ld[k]);
         }
     }
+    // mode k of entry b: square factors of at most 288 bytes are read with
+    // scalar loads into SGPRs (no VGPRs or LDS reads for them); otherwise
+    // from the entry's factors staged in LDS
+    __device__ __forceinline__ void contract(const T *src, T *dst, int outer, int inner, int b,
+                                             int k) const
+    {
+        if constexpr (SQ > 0 && (size_t)SQ * SQ * sizeof(Acc) <= 288) {
+            if ((int)threadIdx.x >= outer * inner) return;
+            Acc a[SQ][SQ];
+            load_factor<T, Acc, SQ, TRANS, true>(a, ent.template factor_g<true>(b, k), sh.ld[k]);
+            contract_fibers_reg<T, Acc, SQ>(src, dst, outer, inner, a);
+        } else {
+            contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, outer, sh.n[k], inner, sh.m[k],
+                                                 facs + sh.fac_off[k],
+                                                 TRANS ? sh.n[k] : sh.m[k]);
+        }
+    }
     // contract the item's modes in LDS, input in buf(in); the result ends in
     // buf(in ^ (modes() & 1))
-    __device__ __forceinline__ void compute(int C, int in) const
+    __device__ __forceinline__ void compute(int b, int C, int in) const
     {
         const int h = sh.h, d = sh.d;
         T *src = buf(in), *dst = buf(in ^ 1);
@@ -183,9 +200,7 @@ struct GenericPasses {
             int inner = C;
             for (int k = 1; k < h; ++k) inner *= sh.n[k];
             for (int q = 0; q < h; ++q) {
-                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, outer, sh.n[q], inner, sh.m[q],
-                                                     facs + sh.fac_off[q],
-                                                     TRANS ? sh.n[q] : sh.m[q]);
+                contract(src, dst, outer, inner, b, q);
                 outer *= sh.m[q];
                 if (q + 1 < h) inner /= sh.n[q + 1];
                 __syncthreads();
@@ -198,9 +213,7 @@ struct GenericPasses {
             int inner = 1;
             for (int k = h + 1; k < d; ++k) inner *= sh.n[k];
             for (int k = h; k < d; ++k) {
-                contract_mode_lds<T, Acc, TRANS, SQ>(src, dst, outer, sh.n[k], inner, sh.m[k],
-                                                     facs + sh.fac_off[k],
-                                                     TRANS ? sh.n[k] : sh.m[k]);
+                contract(src, dst, outer, inner, b, k);
                 outer *= sh.m[k];
                 if (k + 1 < d) inner /= sh.n[k + 1];
                 __syncthreads();
@@ -289,7 +302,7 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
         __syncthreads();
         const bool more = r + 1 < r1;
         if (more) P.load(st, r + 1);
-        P.compute(C, in);
+        P.compute(b, C, in);
         // The next item's input goes into the buffer the result is not in,
         // before this item's stores are issued: waiting for its loads then
         // does not also wait for those stores (the memory counter is in order).
