@@ -107,9 +107,11 @@ struct MixPath {
     static constexpr long long scratch_elems() { return BIG ? E : 0; }
 };
 
-// Largest factor (bytes) held in SGPRs: the kernel keeps more scalar state
-// live than a single-shape kernel does.
-constexpr int kMixSBytes = 128;
+// Largest factor (bytes) held in SGPRs: up to 6 x 6 in FP64.  The kernel keeps
+// more scalar state live than a single-shape kernel, so some of it spills to
+// VGPR lanes; that still beats holding the factor in VGPRs through vector
+// loads, whose waits share the in-order memory counter with the data loads.
+constexpr int kMixSBytes = 288;
 
 // Paths with N <= 6 and at most 24 first-pass values per thread keep the
 // next unit's input in flight through their remaining passes (factor plus
