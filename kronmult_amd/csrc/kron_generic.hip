@@ -124,33 +124,29 @@ struct GenericPasses {
             // rows of C consecutive low indices, nlo_in apart
             const auto s1 = src + c0;
             const FastDiv dc = fast_div(C);
+        // The loads are unconditional (element indices clamped into the
+        // chunk): every item issues the same number, so the wait for one item
+        // can leave the next item's loads in flight.
 #pragma unroll
             for (int k = 0; k < kStagePer; ++k) {
-                const int w = threadIdx.x + k * 256;
-                if (w < cnt) {
-                    int c;
-                    const int hi = split_fiber(w, dc, c);
-                    st.v[k] = s1[(long long)hi * sh.nlo_in + c];
-                }
+                const int w = min((int)threadIdx.x + k * 256, cnt - 1);
+                int c;
+                const int hi = split_fiber(w, dc, c);
+                st.v[k] = s1[(long long)hi * sh.nlo_in + c];
             }
         } else {
             const auto s2 = src + c0 * sh.nlo_in;
 #pragma unroll
-            for (int k = 0; k < kStagePer; ++k) {
-                const int w = threadIdx.x + k * 256;
-                if (w < cnt) st.v[k] = s2[w];
-            }
+            for (int k = 0; k < kStagePer; ++k) st.v[k] = s2[min((int)threadIdx.x + k * 256, cnt - 1)];
         }
         if constexpr (!SHARED) {
             // the item's Y chunk is read now, so its latency overlaps too
             if (phase == 2) {
-                const auto y = to_global(ent.y(b));
+                const auto y = to_global(ent.y(b)) + c0 * sh.nlo_out;
                 const int nout = C * (int)sh.nlo_out;
 #pragma unroll
-                for (int k = 0; k < kStagePer; ++k) {
-                    const int w = threadIdx.x + k * 256;
-                    if (w < nout) st.y[k] = y[c0 * sh.nlo_out + w];
-                }
+                for (int k = 0; k < kStagePer; ++k)
+                    st.y[k] = y[min((int)threadIdx.x + k * 256, nout - 1)];
             }
         }
     }
@@ -279,13 +275,19 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
     const long long per = (items + gridDim.x - 1) / gridDim.x;
     const long long r0 = blockIdx.x * per;
     const long long r1 = min(items, r0 + per);
-    RegStage<T> st;
-    if (r0 < r1) P.load(st, r0);
+    // Two items in flight per workgroup: item r+2 is loaded into the
+    // register stage item r came from while item r+1's loads are landing.
+    RegStage<T> st[2];
+    auto clampr = [&](long long q) { return q < r1 ? q : r1 - 1; };
+    if (r0 < r1) {
+        P.load(st[0], r0);
+        P.load(st[1], clampr(r0 + 1));
+    }
     int cur_b = -1;
-    int in = 0;  // LDS buffer holding the current item's input
-    bool staged = false;
+    int in = 0;                      // LDS buffer holding the current item's input
     const int flip = P.modes() & 1;  // the result lands in buf(in ^ flip)
-    for (long long r = r0; r < r1; ++r) {
+    auto item = [&](long long r, auto slot) __attribute__((always_inline)) {
+        constexpr int S = decltype(slot)::value;  // register stage of item r
         int b, C;
         long long c0;
         P.decode(r, b, c0, C);
@@ -293,31 +295,33 @@ kron_generic_kernel(Entries<T, LAY> ent, GenShape sh, ScratchDesc<T> scr, int ph
             P.stage_factors(b);
             cur_b = b;
         }
-        if (!staged) P.to_lds(st, P.count(C), P.buf(in));
+        if (r == r0) P.to_lds(st[S], P.count(C), P.buf(in));
         RegStage<T> ycur;
         if constexpr (!SHARED) {
 #pragma unroll
-            for (int k = 0; k < kStagePer; ++k) ycur.y[k] = st.y[k];
+            for (int k = 0; k < kStagePer; ++k) ycur.y[k] = st[S].y[k];
         }
         __syncthreads();
-        const bool more = r + 1 < r1;
-        if (more) P.load(st, r + 1);
+        P.load(st[S], clampr(r + 2));  // a repeat of the last item past the end
         P.compute(b, C, in);
         // The next item's input goes into the buffer the result is not in,
         // before this item's stores are issued: waiting for its loads then
         // does not also wait for those stores (the memory counter is in order).
         const int res = in ^ flip, next = res ^ 1;
-        if (more) {
+        if (r + 1 < r1) {
             int nb, nC;
             long long nc0;
             P.decode(r + 1, nb, nc0, nC);
-            P.to_lds(st, P.count(nC), P.buf(next));
+            P.to_lds(st[S ^ 1], P.count(nC), P.buf(next));
         }
-        staged = more;
         P.store(b, c0, C, P.buf(res), ycur);
         in = next;
         // (the barrier at the top of the next item orders these LDS reads
         // before that item's contractions overwrite buf(res))
+    };
+    for (long long r = r0; r < r1; r += 2) {
+        item(r, std::integral_constant<int, 0>{});
+        if (r + 1 < r1) item(r + 1, std::integral_constant<int, 1>{});
     }
 }
 
