@@ -179,6 +179,11 @@ def bench_xbatched(args):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernel_ms": ms_kernel,
         "step_alg_gbs": alg_bytes / (ms * 1e-3) / 1e9,
+        # the kernel's measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+        # profiles/traffic_xbatched_d6n4.json) over its live duration: merging
+        # Y runs removes most of the algorithmic Y read-modify-write, so this,
+        # not roofline.frac, shows how close the kernel runs to HBM bandwidth
+        "kernel_traffic_gbs": (traffic / (ms_kernel * 1e-3) / 1e9) if traffic else None,
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
