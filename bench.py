@@ -239,12 +239,30 @@ def bench_adjoint(args):
     ms_a = _time_steps(lambda: K.kronmult_batched_adjoint(d, n, A, X, dY, dX, dA, W, batch,
                                                           stream=stream),
                        args.steps, args.warmup, stream)
+    # the same through the torch.autograd wrapper: Y = kron_matvec(A, X) (a
+    # fresh zeroed Y, as the kernel accumulates), then Y.backward(dY)
+    from kronmult_amd.torch_ops import kron_matvec
+
+    Aw = A.view(batch, d, n, n).transpose(-1, -2).contiguous().requires_grad_(True)
+    Xw = X.view(batch, E).clone().requires_grad_(True)
+    dYw = dY.view(batch, E)
+
+    def fwd_bwd():
+        Aw.grad = None
+        Xw.grad = None
+        kron_matvec(Aw, Xw).backward(dYw)
+
+    with torch.no_grad():
+        ms_wf = _time_steps(lambda: kron_matvec(Aw, Xw), args.steps, args.warmup, stream)
+    ms_wfb = _time_steps(fwd_bwd, args.steps, args.warmup, stream)
     fwd_bytes = bytes_per_entry(d, n, ptr_layout=False) * batch
     adj_bytes = 8 * (3 * E + 2 * d * n * n) * batch  # X, dY read, dX written, A read, dA written
     print(json.dumps({"workload": "forward + adjoint d=6 n=6 batchCount=4096", "dtype": "f64",
                       "forward_ms": ms_f, "adjoint_ms": ms_a, "adjoint_over_forward": ms_a / ms_f,
                       "forward_alg_gbs": fwd_bytes / (ms_f * 1e-3) / 1e9,
-                      "adjoint_alg_gbs": adj_bytes / (ms_a * 1e-3) / 1e9}), flush=True)
+                      "adjoint_alg_gbs": adj_bytes / (ms_a * 1e-3) / 1e9,
+                      "autograd_forward_ms": ms_wf, "autograd_forward_backward_ms": ms_wfb,
+                      "autograd_backward_ms": ms_wfb - ms_wf}), flush=True)
 
 
 def bench_mixed(args):
