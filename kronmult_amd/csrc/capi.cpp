@@ -53,12 +53,12 @@ long long persistent_blocks(const void *kernel, size_t lds)
     return r;
 }
 
-void *stream_workspace(hipStream_t stream, size_t bytes)
+void *stream_workspace(hipStream_t stream, size_t bytes, bool *fresh)
 {
-    return stream_workspace_slot(stream, bytes, 0);
+    return stream_workspace_slot(stream, bytes, 0, fresh);
 }
 
-void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot)
+void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot, bool *fresh)
 {
     struct Buf {
         void *ptr = nullptr;
@@ -70,6 +70,7 @@ void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot)
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lk(mu);
     Buf &b = bufs[std::make_tuple(dev, stream, slot)];
+    if (fresh) *fresh = false;
     if (b.bytes < bytes) {
         // the old buffer may still be in use by queued work on this stream
         if (b.ptr) {
@@ -84,6 +85,7 @@ void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot)
             return nullptr;
         }
         b.bytes = want;
+        if (fresh) *fresh = true;
     }
     return b.ptr;
 }
@@ -329,7 +331,7 @@ static kron_status plan_create(kron_xplan *out, int d, int n, const T *const *Aa
             return KRON_EHIP;
         }
         st = group_by_target(reinterpret_cast<const void *const *>(pY), batch, p->ws, bytes,
-                             &p->ord.keys, &p->ord.perm, static_cast<hipStream_t>(stream));
+                             true, &p->ord.keys, &p->ord.perm, static_cast<hipStream_t>(stream));
         if (st != KRON_OK) {
             (void)hipFree(p->ws);
             delete p;

@@ -499,12 +499,13 @@ static bool group_order(const KronArgs<T> &a, int N, GroupOrder &ord, hipStream_
     }
     if (a.batch < kGroupMinBatch) return false;
     const size_t bytes = group_workspace_bytes(a.batch);
-    void *ws = stream_workspace(s, bytes);
+    bool fresh = false;
+    void *ws = stream_workspace(s, bytes, &fresh);
     if (!ws) return false;
     const unsigned long long *keys = nullptr;
     const int *perm = nullptr;
-    if (group_by_target(reinterpret_cast<const void *const *>(a.pY), a.batch, ws, bytes, &keys,
-                        &perm, s) != KRON_OK)
+    if (group_by_target(reinterpret_cast<const void *const *>(a.pY), a.batch, ws, bytes, fresh,
+                        &keys, &perm, s) != KRON_OK)
         return false;
     ord.perm = perm;
     ord.keys = keys;

@@ -87,18 +87,21 @@ kron_status launch_kgemm(bool transB, int mm, int nn, int kk, T alpha,
                          const int *ldB, T beta, T *const *C, const int *ldC,
                          int batch, hipStream_t stream);
 
-// Y-target ordering of pointer-layout entries (kron_group.hip).
+// Y-target ordering of pointer-layout entries (kron_group.hip).  The start
+// of the workspace must be zero before the first call on it (pass fresh =
+// true for a new workspace: the call clears it); every call leaves it zero.
 size_t group_workspace_bytes(int batch);
 kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t ws_bytes,
-                            const unsigned long long **keys_sorted, const int **perm,
+                            bool fresh, const unsigned long long **keys_sorted, const int **perm,
                             hipStream_t stream);
 // Device scratch owned by the library, one buffer per (device, stream),
 // grown on demand and kept for the life of the process.  Work on one stream
-// is ordered, so calls on the same stream may reuse it.  nullptr on failure.
-void *stream_workspace(hipStream_t stream, size_t bytes);
+// is ordered, so calls on the same stream may reuse it.  nullptr on failure;
+// *fresh (if given) tells whether the buffer was just allocated.
+void *stream_workspace(hipStream_t stream, size_t bytes, bool *fresh = nullptr);
 // The same with an independent buffer per slot (0: the Y-target sort,
 // 1: pointer tables built on the device).
-void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot);
+void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot, bool *fresh = nullptr);
 
 // Device table out[i] = A + i * fstride, i < count (pbatched factor pointers),
 // in the stream's slot-1 workspace.  nullptr on failure.

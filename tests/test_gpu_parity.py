@@ -383,6 +383,22 @@ def test_xbatched_repeat_calls_accumulate():
     assert np.array_equal(host(Yd), 2 * want)
 
 
+def test_xbatched_grouped_sizes_share_workspace():
+    """Grouped calls of changing sizes on one stream share the library's
+    ordering workspace: its bucket counts must be back at zero after every
+    call, for any bucket count (2^11 .. 2^16), and a grown workspace starts
+    cleared.  70,000 entries exceed one entry per counting thread."""
+    d, n = 4, 5
+    for batch, ny in [(2500, 7), (9000, 300), (2048, 1), (70000, 2000), (2600, 9),
+                      (33000, 5000)]:
+        A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 97, ny, n, batch % 101,
+                                                      kind="int")
+        Y0 = H.integers(ny * n ** d, 3, -5, 5).reshape(ny, -1)
+        got = _gpu_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+        want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+        assert np.array_equal(got, want), batch
+
+
 @pytest.mark.parametrize("d,n,batch,ny", [(6, 4, 3000, 7), (6, 4, 40, 3), (4, 5, 2500, 5),
                                           (6, 6, 5, 2)])
 def test_xbatched_plan(d, n, batch, ny):
