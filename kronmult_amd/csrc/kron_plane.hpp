@@ -153,9 +153,10 @@ struct PlaneWindow {
 // P == 1): GROUPED walks the Y-target order and adds each run's summed
 // result to Y once; otherwise Y is updated per entry (atomically if SHARED).
 // Iteration j of a workgroup is position rbeg + j * rstep.  While entry j is
-// contracted, its successor's X plane is loaded into registers and wave 0
-// pulls the successor's six factor lines into L2 (one vector load per line),
-// so the successor's scalar factor loads find them there.
+// contracted, its successor's X plane is loaded into registers.  In the
+// pointer layout wave 0 also pulls the successor's six factor lines into L2
+// (one vector load per line), so the successor's scalar factor loads find
+// them there (in the batched layout this measured 3% slower, so it is off).
 template <typename T, typename Acc, Layout LAY, bool SHARED, bool TRANS, bool GROUPED>
 __global__ void __launch_bounds__(256)
 kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
@@ -240,7 +241,7 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
 #pragma unroll
             for (int q = 0; q < 4; ++q) LA[272 * i + 17 * q] = T(v[i][q]);
         __syncthreads();
-        if (more && w == 0) {
+        if (PlaneWindow<T, LAY, GROUPED>::USED && more && w == 0) {
             const PlaneMeta nm = win.at(ent, j + 1, wb, rbeg, rstep);
             unsigned long long p = nm.f[0];
 #pragma unroll
