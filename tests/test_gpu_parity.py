@@ -399,6 +399,36 @@ def test_xbatched_grouped_sizes_share_workspace():
         assert np.array_equal(got, want), batch
 
 
+def test_xbatched_grouped_long_chunks_by_counting():
+    """150,000 entries at d=6 n=4: every workgroup walks ~147 positions, so its
+    window of entry addresses (32 iterations) is refilled several times.  The
+    entries combine 2 factor sets with 3 X vectors, so Y is known exactly from
+    six oracle evaluations and the count of each (Y target, combination):
+    an entry given another entry's X or factors changes those counts."""
+    d, n, batch, ny = 6, 4, 150000, 64
+    E = n ** d
+    A_sets = H.integers(2 * d * n * n, 41, -2, 2).reshape(2 * d, n * n)
+    X_pool = H.integers(3 * E, 42, -8, 8).reshape(3, E)
+    a_of = (H.splitmix64(batch, 43) % np.uint64(2)).astype(np.int64)
+    x_of = (H.splitmix64(batch, 44) % np.uint64(3)).astype(np.int64)
+    y_of = (H.splitmix64(batch, 45) % np.uint64(ny)).astype(np.int64)
+    Y0 = H.integers(ny * E, 46, -5, 5).reshape(ny, E)
+    Ad, Xd, Yd = dev(A_sets), dev(X_pool), dev(Y0)
+    pA = ptrs(Ad, (a_of[:, None] * d + np.arange(d)).reshape(-1))
+    pX, pY = ptrs(Xd, x_of), ptrs(Yd, y_of)
+    K.kronmult6_xbatched(n, pA, n, pX, pY, pX, batch)
+    got = host(Yd)
+    want = Y0.astype(np.float64)
+    for a in range(2):
+        for x in range(3):
+            v = _oracle_xbatched(d, n, n, A_sets[a * d:(a + 1) * d], X_pool[x:x + 1],
+                                 np.zeros(1, dtype=np.int64), np.zeros((1, E)),
+                                 np.zeros(1, dtype=np.int64))[0]
+            cnt = np.bincount(y_of[(a_of == a) & (x_of == x)], minlength=ny)
+            want += cnt[:, None].astype(np.float64) * v[None, :]
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("d,n,batch,ny", [(6, 4, 3000, 7), (6, 4, 40, 3), (4, 5, 2500, 5),
                                           (6, 6, 5, 2)])
 def test_xbatched_plan(d, n, batch, ny):
