@@ -162,9 +162,16 @@ kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
         else
             return to_global(ent.x(b));
     };
+    // Pass 2 walks the entries last to first, so it starts on the W slices
+    // pass 1 wrote last (the ones the Infinity Cache may still hold); about
+    // 1% faster at d=3 n=32 and d=4 n=16.
+    auto entry_of = [&](long long it) -> int {
+        const int q = (int)(it / NT);
+        return PASS == 2 ? batch - 1 - q : q;
+    };
     T v[PER];
     auto fetch = [&](long long it) {
-        const int b = (int)(it / NT), k = (int)(it % NT);
+        const int b = entry_of(it), k = (int)(it % NT);
         const auto src = src_of(b);
 #pragma unroll
         for (int q = 0; q < PER; ++q) v[q] = src[gofs(k, tid + q * 256)];
@@ -172,7 +179,7 @@ kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
     long long it = blockIdx.x;
     if (it < items) fetch(it);
     for (; it < items; it += gridDim.x) {
-        const int b = (int)(it / NT), k = (int)(it % NT);
+        const int b = entry_of(it), k = (int)(it % NT);
 #pragma unroll
         for (int q = 0; q < PER; ++q) tile[mfma_pos<T, N>(tid + q * 256)] = v[q];
         __syncthreads();
