@@ -265,7 +265,17 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
 #pragma unroll
             for (int q = 0; q < 4; ++q) LB[i + 4 * q] = T(v[i][q]);
         __syncthreads();
-        // pass C: modes 3 and 1, into Y
+        // pass C: modes 3 and 1, into Y.  A private Y (read-modify-write) is
+        // read before the pass's math, so the loads overlap it.
+        constexpr bool RMW = !GROUPED && !SHARED;
+        T yv[RMW ? 4 : 1][4];
+        if constexpr (RMW) {
+            const auto y = to_global(reinterpret_cast<T *>(cur.y)) + eC;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) yv[i][q] = y[64 * i + 1024 * q];
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -297,11 +307,6 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) atomic_add(y + (64 * i + 1024 * q), T(v[i][q]));
             } else {
-                T yv[4][4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) yv[i][q] = y[64 * i + 1024 * q];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
