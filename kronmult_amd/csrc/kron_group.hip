@@ -91,10 +91,12 @@ kron_group_count(const unsigned long long *pY, int n, int bits, int *counts, uns
     __shared__ int lused[kCountThreads];
     __shared__ int lcount, lbase, last, cursor;
     const int t = threadIdx.x, lane = t & 63;
-    for (int b0 = blockIdx.x * kCountThreads; b0 < n; b0 += gridDim.x * kCountThreads) {
+    // 64-bit positions: b0 + t and the stride may pass INT_MAX near the end
+    for (long long b0 = (long long)blockIdx.x * kCountThreads; b0 < n;
+         b0 += (long long)gridDim.x * kCountThreads) {
         if (t == 0) lcount = 0;
         __syncthreads();
-        const int b = b0 + t;
+        const long long b = b0 + t;
         if (b < n) {
             const int k = (int)group_bucket(pY[b], bits);
             const int r = atomicAdd(counts + k, 1);
@@ -160,10 +162,11 @@ __global__ void __launch_bounds__(256)
 kron_group_scatter(const unsigned long long *pY, int n, int bits, const int *rank,
                    const int *base, int *perm, unsigned long long *keys)
 {
-    for (int b = blockIdx.x * 256 + threadIdx.x; b < n; b += gridDim.x * 256) {
+    for (long long b = (long long)blockIdx.x * 256 + threadIdx.x; b < n;
+         b += (long long)gridDim.x * 256) {
         const unsigned long long p = pY[b];
         const int pos = base[group_bucket(p, bits)] + rank[b];
-        perm[pos] = b;
+        perm[pos] = (int)b;
         keys[pos] = p;
     }
 }
