@@ -253,13 +253,18 @@ constexpr bool fast_pipelined()
 // positions [w*chunk, (w+1)*chunk) of the Y-target order.
 
 // Grouping needs one entry per workgroup and the mode-1 outputs of a thread
-// (the running Y sum) within 32 VGPRs.
+// (the running Y sum) in registers: 32 VGPRs, or 80 / 160 when the entry's
+// LDS already limits a CU to two workgroups / one (a wave may then hold 256
+// / 512 VGPRs).  Without it every element of every entry is an atomic add:
+// at d=6 n=5, 65,536 entries, a billion of them.
 template <typename T, int D, int N>
 constexpr bool fast_groupable()
 {
     using Geo = FastGeom<D, N>;
+    constexpr size_t lds = Geo::lds_bytes(sizeof(T));
+    constexpr size_t budget = lds > 81920 ? 640 : lds > 40960 ? 320 : 128;
     return D >= 2 && Geo::P == 1 &&
-           (size_t)((Geo::F + Geo::G - 1) / Geo::G) * N * sizeof(T) <= 128;
+           (size_t)((Geo::F + Geo::G - 1) / Geo::G) * N * sizeof(T) <= budget;
 }
 
 // Persistent workgroups walk the batch with stride gridDim.x * P.  With

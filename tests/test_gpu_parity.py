@@ -345,7 +345,7 @@ def test_kgemm_shared_c_atomic():
 # (2, 8), (2, 2)) are not grouped: their per-element atomics cost less than the
 # ordering's fixed ~20 us; they run here at the same batch sizes
 @pytest.mark.parametrize("d,n", [(6, 4), (5, 4), (5, 5), (4, 5), (4, 4), (3, 4), (3, 8),
-                                 (2, 8), (2, 2), (4, 6), (6, 3)])
+                                 (2, 8), (2, 2), (4, 6), (6, 3), (6, 5), (5, 6), (5, 7)])
 @pytest.mark.parametrize("ny", [1, 3, 257, 0])
 def test_xbatched_grouped_integer_bit_exact(d, n, ny):
     """Entries sorted by Y target and merged in registers; ny = 0 means every
