@@ -33,20 +33,20 @@ int max_lds_bytes()
     return cached[dev];
 }
 
-long long persistent_blocks(const void *kernel, size_t lds)
+long long persistent_blocks(const void *kernel, size_t lds, int threads)
 {
     static std::mutex mu;
     static std::map<std::pair<const void *, int>, long long> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_pair(kernel, dev * 1048576 + (int)lds);
+    auto key = std::make_pair(kernel, (dev * 1048576 + (int)lds) * 2 + (threads > 256));
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int cus = 0, per_cu = 0;
     long long r = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) == hipSuccess &&
         cus > 0 && per_cu > 0)
         r = (long long)cus * per_cu;
     cache[key] = r;

@@ -17,19 +17,27 @@
 // reference's (1, 2, ..., D), which changes only the rounding.
 //
 // Small entries share a 256-thread workgroup: G = pow2ceil(N^(D-1)) threads
-// (at most 256) per entry, 256/G entries per workgroup.
+// (at most 256) per entry, 256/G entries per workgroup; large ones (see
+// FastGeom) take a 1,024-thread workgroup each.
 #pragma once
 
 #include "kron_device.hpp"
 
 namespace kron {
 
-template <int D, int N>
+// An entry whose LDS alone exceeds 80 KB leaves room for one workgroup per
+// CU; with 256 threads that is 4 waves, too few to hide the passes' LDS and
+// HBM latency.  Such entries get 1,024 threads when the factor fits the
+// scalar registers (a thread then needs well under the 128 VGPRs of four
+// waves per SIMD), else 512 (two waves per SIMD, 256 VGPRs).
+template <typename T, int D, int N>
 struct FastGeom {
     static constexpr int E = cpow(N, D);        // elements per entry
     static constexpr int F = E / N;              // fibers per mode
-    static constexpr int G = F >= 256 ? 256 : pow2ceil(F);  // threads per entry
-    static constexpr int P = 256 / G;            // entries per workgroup
+    static constexpr bool BIG = (size_t)E * sizeof(T) > 81920 && F >= 1024;
+    static constexpr int THREADS = !BIG ? 256 : (size_t)N * N * sizeof(T) <= 200 ? 1024 : 512;
+    static constexpr int G = F >= THREADS ? THREADS : pow2ceil(F);  // threads per entry
+    static constexpr int P = THREADS / G;        // entries per workgroup
     static constexpr size_t lds_bytes(size_t elem) { return (size_t)P * E * elem; }
 };
 
@@ -245,7 +253,7 @@ struct XFibers {
 template <typename T, int D, int N>
 constexpr bool fast_pipelined()
 {
-    using X = XFibers<T, D, N, FastGeom<D, N>::G>;
+    using X = XFibers<T, D, N, FastGeom<T, D, N>::G>;
     return D >= 2 && (size_t)X::IT * N * sizeof(T) <= 128;
 }
 
@@ -260,9 +268,14 @@ constexpr bool fast_pipelined()
 template <typename T, int D, int N>
 constexpr bool fast_groupable()
 {
-    using Geo = FastGeom<D, N>;
+    using Geo = FastGeom<T, D, N>;
     constexpr size_t lds = Geo::lds_bytes(sizeof(T));
-    constexpr size_t budget = lds > 81920 ? 640 : lds > 40960 ? 320 : 128;
+    // 1,024 threads: four waves per SIMD, 128 VGPRs each; 512: two, 256
+    constexpr size_t budget = Geo::THREADS == 1024 ? 160
+                              : Geo::THREADS == 512 ? 320
+                              : lds > 81920        ? 640
+                              : lds > 40960        ? 320
+                                                   : 128;
     return D >= 2 && Geo::P == 1 &&
            (size_t)((Geo::F + Geo::G - 1) / Geo::G) * N * sizeof(T) <= budget;
 }
@@ -275,10 +288,10 @@ constexpr bool fast_groupable()
 // and added to Y once per run ("batched atomicAdd merges").
 template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool TRANS,
           bool GROUPED>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((FastGeom<T, D, N>::THREADS))
 kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
 {
-    using Geo = FastGeom<D, N>;
+    using Geo = FastGeom<T, D, N>;
     constexpr int E = Geo::E;
     constexpr int G = Geo::G;
     constexpr int P = Geo::P;
@@ -481,7 +494,8 @@ template <typename T, typename Acc, int D, int N, Layout LAY, bool SHARED, bool 
 kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipStream_t stream,
                             const GroupOrder *ord)
 {
-    using Geo = FastGeom<D, N>;
+    using Geo = FastGeom<T, D, N>;
+    constexpr int THREADS = fast_uses_plane<D, N>() ? 256 : Geo::THREADS;
     const size_t lds = (D == 1)                    ? 0
                        : fast_uses_plane<D, N>() ? (size_t)kPlaneLds * sizeof(T)
                                                  : Geo::lds_bytes(sizeof(T));
@@ -506,7 +520,7 @@ kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipS
             return KRON_EHIP;
         }
     }
-    const long long resident = persistent_blocks(kern, lds);
+    const long long resident = persistent_blocks(kern, lds, THREADS);
     GroupOrder o{nullptr, nullptr, 0};
     long long nblocks = ((long long)batch + Geo::P - 1) / Geo::P;
     if (grouped) {
@@ -523,15 +537,15 @@ kron_status launch_fast_one(const Entries<T, LAY> &ent, int lda, int batch, hipS
     if constexpr (CAN_GROUP) {
         if (grouped)
             hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, true>()),
-                               dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
+                               dim3((unsigned)nblocks), dim3(THREADS), lds, stream, ent, lda, batch,
                                o);
         else
             hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>()),
-                               dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch,
+                               dim3((unsigned)nblocks), dim3(THREADS), lds, stream, ent, lda, batch,
                                o);
     } else {
         hipLaunchKernelGGL((fast_kernel_for<T, Acc, D, N, LAY, SHARED, TRANS, false>()),
-                           dim3((unsigned)nblocks), dim3(256), lds, stream, ent, lda, batch, o);
+                           dim3((unsigned)nblocks), dim3(THREADS), lds, stream, ent, lda, batch, o);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -130,6 +130,6 @@ void set_last_hip_error(int code);
 int max_lds_bytes();
 // Workgroups of `kernel` (256 threads, `lds` dynamic bytes) that fit on the
 // current device at once: CUs x resident blocks per CU (0 if unknown).
-long long persistent_blocks(const void *kernel, size_t lds);
+long long persistent_blocks(const void *kernel, size_t lds, int threads = 256);
 
 }  // namespace kron
