@@ -578,6 +578,8 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = launch_mfma<T>(a.d, a.n[0], e, a.W, a.batch, a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
+    // d = 1: a batched matrix-vector product, one lane per output row
+    if (a.d == 1 && !a.trans) return launch_gemv(a);
     GenShape sh{};
     if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};

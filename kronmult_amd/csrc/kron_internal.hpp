@@ -65,6 +65,11 @@ struct KronArgs {
     const GroupOrder *order = nullptr;
 };
 
+// d = 1 entries (a batched matrix-vector product, kron_gemv.hip): KRON_ENOTSUP
+// for other d or transposed factors.
+template <typename T>
+kron_status launch_gemv(const KronArgs<T> &a);
+
 // Launch the forward kernel family.  Returns KRON_OK, KRON_ENOTSUP when no
 // kernel covers the shape, KRON_EWORKSPACE when scratch is needed but absent,
 // or KRON_EHIP on a launch failure.

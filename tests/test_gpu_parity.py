@@ -258,6 +258,47 @@ def test_vbatched_rectangular(seed):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+# --- d = 1 beyond the fused kernels' n <= 8: the matrix-vector kernel -------
+
+@pytest.mark.parametrize("n", [9, 16, 33, 64, 100])
+def test_batched_d1_gemv(n):
+    """One lane per output row; n = 100 takes two row steps per lane."""
+    batch = 37
+    A = H.integers(batch * n * n, 61, -2, 2).reshape(batch, 1, n * n)
+    X = H.integers(batch * n, 62, -8, 8).reshape(batch, n)
+    Y0 = H.integers(batch * n, 63, -9, 9).reshape(batch, n)
+    assert np.array_equal(_gpu_batched(1, n, A, X, Y0), _oracle_batched(1, n, A, X, Y0))
+
+
+@pytest.mark.parametrize("n,lda,batch", [(12, 12, 500), (12, 15, 500), (70, 71, 300),
+                                         (9, 9, 150000)])
+def test_xbatched_d1_gemv_shared_y(n, lda, batch):
+    """Shared Y targets (atomic rows), a padded leading dimension, and a batch
+    that takes several grid-stride steps."""
+    ny = 7
+    A_pool, X_pool, x_map, y_map = _xbatched_case(1, n, batch, 30, ny, lda, 9, kind="int")
+    Y0 = np.zeros((ny, n))
+    got = _gpu_xbatched(1, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(1, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("m,n", [(7, 13), (70, 5), (1, 40), (64, 64)])
+def test_vbatched_d1_rectangular(m, n):
+    batch, ny = 20, 3
+    A_pool, X_pool, y_map = _vb_case([m], [n], batch, m * 100 + n, ny)
+    Y0 = np.zeros((ny, m))
+    Wcap = 2 * max(m, n) * 8
+    Yw = Y0.copy()
+    oracle.vbatched(O, [m], [n], A_pool, np.arange(batch).reshape(batch, 1), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched([m], [n], ptrs(Ad, np.arange(batch)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
+
+
 def test_vbatched_large_two_pass_and_workspace_rule():
     m, n = [7, 6, 9, 8, 7], [8, 7, 6, 7, 8]
     batch = 3
