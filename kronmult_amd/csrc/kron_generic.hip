@@ -583,6 +583,11 @@ kron_status launch_kron(const KronArgs<T> &a)
     }
     // d = 1: a batched matrix-vector product, one lane per output row
     if (a.d == 1 && !a.trans) return launch_gemv(a);
+    // d = 2, extents up to 32: a wave per entry
+    if (a.d == 2 && !a.trans) {
+        kron_status st = launch_d2(a);
+        if (st != KRON_ENOTSUP) return st;
+    }
     GenShape sh{};
     if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};

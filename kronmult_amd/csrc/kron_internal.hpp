@@ -70,6 +70,11 @@ struct KronArgs {
 template <typename T>
 kron_status launch_gemv(const KronArgs<T> &a);
 
+// d = 2 entries with factor extents up to 32 (kron_d2.hip): KRON_ENOTSUP for
+// other shapes or transposed factors.
+template <typename T>
+kron_status launch_d2(const KronArgs<T> &a);
+
 // Launch the forward kernel family.  Returns KRON_OK, KRON_ENOTSUP when no
 // kernel covers the shape, KRON_EWORKSPACE when scratch is needed but absent,
 // or KRON_EHIP on a launch failure.

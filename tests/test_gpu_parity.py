@@ -299,6 +299,46 @@ def test_vbatched_d1_rectangular(m, n):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+# --- d = 2 with extents up to 32 beyond the fused kernels: a wave per entry --
+
+@pytest.mark.parametrize("n", [9, 12, 15, 20, 31])
+def test_batched_d2_wave(n):
+    batch = 41
+    A = H.integers(batch * 2 * n * n, 71, -2, 2).reshape(batch, 2, n * n)
+    X = H.integers(batch * n * n, 72, -8, 8).reshape(batch, n * n)
+    Y0 = H.integers(batch * n * n, 73, -9, 9).reshape(batch, n * n)
+    assert np.array_equal(_gpu_batched(2, n, A, X, Y0), _oracle_batched(2, n, A, X, Y0))
+
+
+@pytest.mark.parametrize("n,lda,batch", [(12, 12, 3000), (10, 13, 700), (32, 32, 300)])
+def test_xbatched_d2_wave_shared_y(n, lda, batch):
+    ny = 5
+    A_pool, X_pool, x_map, y_map = _xbatched_case(2, n, batch, 40, ny, lda, 19, kind="int")
+    Y0 = np.zeros((ny, n * n))
+    got = _gpu_xbatched(2, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(2, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("m,n", [([7, 13], [11, 5]), ([3, 2], [5, 4]), ([32, 9], [10, 32]),
+                                 ([1, 17], [20, 1])])
+def test_vbatched_d2_rectangular(m, n):
+    batch, ny = 23, 4
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, sum(m) * 10 + sum(n), ny)
+    sy = m[0] * m[1]
+    Y0 = np.zeros((ny, sy))
+    _, sxw = _sizes(m, n)
+    Wcap = 2 * sxw * 8
+    Yw = Y0.copy()
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * 2).reshape(batch, 2), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * 2)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
+
+
 def test_vbatched_large_two_pass_and_workspace_rule():
     m, n = [7, 6, 9, 8, 7], [8, 7, 6, 7, 8]
     batch = 3
