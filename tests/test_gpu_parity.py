@@ -339,6 +339,31 @@ def test_vbatched_d2_rectangular(m, n):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+def test_batched_d2_wave_many_entries():
+    """60,000 entries: several grid-stride steps of the wave-per-entry kernel."""
+    n, batch = 10, 60000
+    A = H.integers(batch * 2 * n * n, 81, -2, 2).reshape(batch, 2, n * n)
+    X = H.integers(batch * n * n, 82, -4, 4).reshape(batch, n * n)
+    Y0 = np.zeros((batch, n * n))
+    assert np.array_equal(_gpu_batched(2, n, A, X, Y0), _oracle_batched(2, n, A, X, Y0))
+
+
+@pytest.mark.parametrize("d,n,batch", [(1, 20, 50), (1, 65, 3), (1, 9, 1), (2, 12, 30),
+                                       (2, 25, 7)])
+def test_small_d_kernels_f32(d, n, batch):
+    """FP32 through the d = 1 (matrix-vector) and d = 2 (wave per entry)
+    kernels against the FP64 oracle on the same rounded inputs."""
+    E = n ** d
+    A = H.uniform(batch * d * n * n, 91).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, 92).reshape(batch, E)
+    Y0 = H.uniform(batch * E, 93).reshape(batch, E)
+    got = _gpu_batched(d, n, A.astype(np.float32), X.astype(np.float32), Y0.astype(np.float32))
+    want = _oracle_batched(d, n, A.astype(np.float32), X.astype(np.float32),
+                           Y0.astype(np.float32))
+    # d n-term FP32 dot products: k = d*n roundings of 2^-24 (x8 for partial sums)
+    assert H.rel_err(got, want) < 8 * d * n * 2.0 ** -24
+
+
 def test_vbatched_large_two_pass_and_workspace_rule():
     m, n = [7, 6, 9, 8, 7], [8, 7, 6, 7, 8]
     batch = 3
