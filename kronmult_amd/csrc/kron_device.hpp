@@ -293,9 +293,55 @@ __device__ __forceinline__ void contract_fibers_reg(const T *__restrict__ src,
     }
 }
 
+// Square N = 9..15: the factor (N*N values) no longer fits one thread's
+// registers, so it is cut into C = ceil(N / R) blocks of R rows.  Thread t
+// keeps rows [c R, c R + R) (c = t mod C) in registers for the whole mode and
+// walks the fibers t / C, t / C + S, ... (S = floor(256 / C) fibers per
+// sweep; threads from S C on sit the mode out): each input element read from
+// LDS feeds R FMAs, where one work item per output element costs two LDS
+// reads (factor and input) per FMA.  Rows past N (the last block when R does
+// not divide N) are computed from a clamped duplicate row and never stored.
+template <typename T, typename Acc, bool TRANS, int N, int R>
+__device__ __forceinline__ void contract_rowblocks(const T *__restrict__ src,
+                                                   T *__restrict__ dst, int outer, int inner,
+                                                   const T *__restrict__ Al)
+{
+    constexpr int C = (N + R - 1) / R;  // row blocks: blockDim / C fibers per sweep
+    const int step = (int)blockDim.x / C;
+    if ((int)threadIdx.x >= step * C) return;
+    const int c = (int)threadIdx.x % C;
+    const int i0 = c * R;
+    Acc a[R][N];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = min(i0 + r, N - 1);
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[r][j] = TRANS ? Acc(Al[j + i * N]) : Acc(Al[i + j * N]);
+    }
+    const int fibers = outer * inner;
+    const FastDiv di = fast_div(inner);
+    for (int f = (int)threadIdx.x / C; f < fibers; f += step) {
+        int ii;
+        const int o = split_fiber(f, di, ii);
+        const int base = fiber_base(o, ii, N * inner);
+        const T *s = src + base;
+        T *dd = dst + base;
+        Acc in[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) in[j] = Acc(s[j * inner]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Acc acc = a[r][0] * in[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[r][j], in[j], acc);
+            if (i0 + r < N) dd[(i0 + r) * inner] = T(acc);
+        }
+    }
+}
+
 // Square variant with compile-time N: the factor is read once from LDS into
-// registers (N*N values), then contract_fibers_reg.  SQ == 0 selects
-// contract_fibers (runtime extents).
+// registers (N*N values), then contract_fibers_reg; N = 9..15 goes through
+// contract_rowblocks.  SQ == 0 selects contract_fibers (runtime extents).
 template <typename T, typename Acc, bool TRANS, int SQ>
 __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *__restrict__ dst,
                                                   int outer, int n, int inner, int m,
@@ -303,6 +349,11 @@ __device__ __forceinline__ void contract_mode_lds(const T *__restrict__ src, T *
 {
     if constexpr (SQ == 0) {
         contract_fibers<T, Acc, TRANS, 8>(src, dst, outer, n, inner, m, Al, ldA);
+    } else if constexpr (SQ > 8) {
+        // three-row blocks measured faster at N = 11, 12 (d = 3..4, FP64), two
+        // rows elsewhere (the registers of a third row cost occupancy)
+        constexpr int R = (SQ == 11 || SQ == 12) ? 3 : 2;
+        contract_rowblocks<T, Acc, TRANS, SQ, R>(src, dst, outer, inner, Al);
     } else {
         constexpr int N = SQ;
         if ((int)threadIdx.x >= outer * inner) return;

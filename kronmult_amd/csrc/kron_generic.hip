@@ -449,7 +449,11 @@ static kron_status launch_generic_sq(const Entries<T, LAY> &ent, const GenShape 
 }
 
 // Square shapes with n <= 8 get the compile-time contraction (factor in
-// registers); everything else the runtime-extent one.
+// registers); n = 9..15 the row-block one (contract_rowblocks) in FP32, and
+// in FP64 at d >= 4 and at d = 3 for n = 9, 12, where it measured 6-16%
+// faster (batched; at FP64 d = 3, n = 10, 13 it was 9-11% slower, at n = 11,
+// 15 even; in FP32 it was 3-30% faster at every d = 3 n); everything else the
+// runtime-extent one.
 template <typename T, Layout LAY, bool SHARED, bool TRANS>
 static kron_status launch_generic_t(const Entries<T, LAY> &ent, const GenShape &sh,
                                     const ScratchDesc<T> &scr, int group, int batch,
@@ -457,12 +461,15 @@ static kron_status launch_generic_t(const Entries<T, LAY> &ent, const GenShape &
 {
     bool square = true;
     for (int k = 0; k < sh.d; ++k) square = square && sh.m[k] == sh.n[0] && sh.n[k] == sh.n[0];
-    const int N = square ? sh.n[0] : 0;
+    const int n0 = sh.n[0];
+    const bool rowblocks = sizeof(T) == 4 || sh.d >= 4 || n0 == 9 || n0 == 12;
+    const int N = square && (n0 <= 8 || rowblocks) ? n0 : 0;
     switch (N) {
 #define KRON_SQ(NN)                                                                          \
     case NN:                                                                                 \
         return launch_generic_sq<T, LAY, SHARED, TRANS, NN>(ent, sh, scr, group, batch, stream);
         KRON_SQ(2) KRON_SQ(3) KRON_SQ(4) KRON_SQ(5) KRON_SQ(6) KRON_SQ(7) KRON_SQ(8)
+        KRON_SQ(9) KRON_SQ(10) KRON_SQ(11) KRON_SQ(12) KRON_SQ(13) KRON_SQ(14) KRON_SQ(15)
 #undef KRON_SQ
     default:
         return launch_generic_sq<T, LAY, SHARED, TRANS, 0>(ent, sh, scr, group, batch, stream);

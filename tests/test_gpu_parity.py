@@ -299,6 +299,51 @@ def test_vbatched_d1_rectangular(m, n):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+# --- square n = 9..15 at d >= 3: the generic kernel's row-block contraction --
+
+@pytest.mark.parametrize("d,n", [(3, n) for n in range(9, 16)] +
+                         [(4, n) for n in range(9, 16)] + [(5, 9)])
+def test_batched_rowblocks(d, n):
+    """d >= 4 covers every row-block geometry (two-row blocks, three-row
+    blocks at n = 11, 12; a partial last block at n = 9, 11, 13, 15) in two
+    passes through W; d = 3 the single-pass row blocks (n = 9, 12) and the
+    runtime-extent kernel (the other n).  Integer values: exact comparison."""
+    batch = 5 if d == 3 else 2
+    E = n ** d
+    A = H.integers(batch * d * n * n, 101 + n, -2, 2).reshape(batch, d, n * n)
+    X = H.integers(batch * E, 102 + n, -4, 4).reshape(batch, E)
+    Y0 = H.integers(batch * E, 103, -9, 9).reshape(batch, E)
+    assert np.array_equal(_gpu_batched(d, n, A, X, Y0), _oracle_batched(d, n, A, X, Y0))
+
+
+@pytest.mark.parametrize("d,n,lda,batch", [(3, 12, 14, 600), (3, 9, 9, 300), (3, 15, 15, 200),
+                                           (4, 10, 11, 40)])
+def test_xbatched_rowblocks_shared_y(d, n, lda, batch):
+    """Shared Y targets (atomic adds), padded leading dimensions."""
+    ny = 6
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 25, ny, lda, 23, kind="int")
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("d,n", [(3, 10), (3, 11), (3, 14), (4, 12)])
+def test_batched_rowblocks_f32(d, n):
+    """FP32 takes the row blocks at every n = 9..15 (d = 3: single pass and,
+    from n = 13, two passes)."""
+    batch = 4
+    E = n ** d
+    A = H.uniform(batch * d * n * n, 111).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, 112).reshape(batch, E)
+    Y0 = H.uniform(batch * E, 113).reshape(batch, E)
+    got = _gpu_batched(d, n, A.astype(np.float32), X.astype(np.float32), Y0.astype(np.float32))
+    want = _oracle_batched(d, n, A.astype(np.float32), X.astype(np.float32),
+                           Y0.astype(np.float32))
+    # d n-term FP32 dot products: k = d*n roundings of 2^-24 (x8 for partial sums)
+    assert H.rel_err(got, want) < 8 * d * n * 2.0 ** -24
+
+
 # --- d = 2 with extents up to 32 beyond the fused kernels: a wave per entry --
 
 @pytest.mark.parametrize("n", [9, 12, 15, 20, 31])
