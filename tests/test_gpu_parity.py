@@ -430,6 +430,27 @@ def test_vbatched_large_two_pass_and_workspace_rule():
     assert e.value.status == K.KRON_EWORKSPACE
 
 
+@pytest.mark.parametrize("nn,d,slots", [(12, 4, 2), (9, 3, 1), (15, 4, 3), (11, 4, 1)])
+def test_vbatched_square_rowblocks(nn, d, slots):
+    """Square vbatched entries at n = 9..15 take the row-block contraction;
+    the workspace holds `slots` scratch slots, so two-pass shapes run in groups
+    of that many entries."""
+    m = n = [nn] * d
+    batch, ny = 7, 3
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, nn, ny)
+    _, sxw = _sizes(m, n)
+    Y0 = np.zeros((ny, nn ** d))
+    Yw = Y0.copy()
+    Wcap = slots * 2 * sxw * 8
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * d).reshape(batch, d), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.empty(Wcap // 8, dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * d)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL, (nn, d)
+
+
 def test_vbatched_square_headline_shape():
     m = n = [4] * 6
     batch = 100
