@@ -424,6 +424,42 @@ def bench_large_n(args):
                       "cases": rows}), flush=True)
 
 
+def bench_shapes(args):
+    """Batched shapes beyond the fused kernels, one JSON line each: d = 1 from
+    n = 8 (matrix-vector kernel), d = 2 at n = 9..32 (a wave per entry), d = 3
+    and 4 at n = 9..15 (generic kernel; row blocks of the factor in registers
+    where they measured faster), FP64 and FP32.  2^25 / n^d entries per shape
+    (about 800 MB of X, Y and W in FP64); uniform [0, 1) data from torch's
+    generator; algorithmic bytes are X + Y read-modify-write + the factors."""
+    import torch
+    import kronmult_amd as K
+    dev = "cuda:0"
+    stream = torch.cuda.current_stream()
+    shapes = [(1, 16), (1, 64), (2, 12), (2, 24), (2, 32)] + \
+             [(3, n) for n in (9, 10, 11, 12, 13, 14, 15)] + \
+             [(4, n) for n in (9, 10, 11, 12, 13, 14, 15)]
+    torch.manual_seed(2024)
+    for dt, tdt in (("f64", torch.float64), ("f32", torch.float32)):
+        w = 8 if dt == "f64" else 4
+        for d, n in shapes:
+            E = n ** d
+            batch = max(1, (1 << 25) // E)
+            A = torch.rand(batch * d * n * n, dtype=tdt, device=dev)
+            X = torch.rand(batch * E, dtype=tdt, device=dev)
+            Y = torch.zeros(batch * E, dtype=tdt, device=dev)
+            W = torch.empty(batch * E, dtype=tdt, device=dev)
+            ms = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, W, batch, stream=stream),
+                             args.steps, args.warmup, stream)
+            alg = w * (3 * E + d * n * n) * batch
+            print(json.dumps({"workload": "kronmult_batched shape", "d": d, "n": n,
+                              "dtype": dt, "batch": batch, "ms": round(ms, 4),
+                              "entries_per_s": batch / (ms * 1e-3),
+                              "alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+                              "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}),
+                  flush=True)
+            del A, X, Y, W
+
+
 def bench_copy(args):
     """Device-to-device copy of 2 GiB: the achievable HBM read+write rate."""
     import torch
@@ -464,6 +500,8 @@ def main():
         bench_mixed(args)
     elif args.workload == "copy":
         bench_copy(args)
+    elif args.workload == "shapes":
+        bench_shapes(args)
     else:
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)

@@ -11,7 +11,12 @@
 // (P = 64 / m2 row groups; the factor's column j2 is contiguous over k2:
 // coalesced) and then Y's column k2 the same way from T in LDS.  The four
 // waves of a workgroup step through entries together (workgroup barriers
-// order each wave's LDS slice between entries).
+// order each wave's LDS slice between entries).  The factors are copied into
+// the slice too.  When four such slices would pass 32 KB (square extents
+// above 16 in FP64, above 22 in FP32), the whole workgroup takes one entry
+// instead (P = 256 / m2 row groups), its factors still in LDS: read from
+// global memory inside the FMA loops, their latency dominated (d2n24 FP64:
+// 0.7 TB/s with a wave per entry and the factors in global memory).
 #include <algorithm>
 
 #include "kron_device.hpp"
@@ -22,8 +27,8 @@ namespace kron {
 struct D2Shape {
     int m1, n1, m2, n2;  // A1: m1 x n1 (slow index), A2: m2 x n2 (fast index)
     int ld1, ld2;        // leading dimensions of A1, A2
-    int slice;           // LDS elements per wave: X (n1 n2), T (n1 m2)[, A1, A2]
-    int fac_lds;         // 1: the factors are copied into the slice too
+    int slice;           // LDS elements per slice: X (n1 n2), T (n1 m2), A1, A2
+    int per;             // entries per workgroup: 4 (a wave and a slice each) or 1
 };
 
 template <typename T, Layout LAY, bool SHARED>
@@ -31,63 +36,49 @@ __global__ void __launch_bounds__(256)
 kron_d2_kernel(Entries<T, LAY> ent, D2Shape sh, long long batch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int tpe = 256 / sh.per;  // threads per entry: 64 or 256
+    const int lane = threadIdx.x & (tpe - 1), wid = threadIdx.x / tpe;
     T *const xs = reinterpret_cast<T *>(kron_smem) + wid * sh.slice;
     T *const ts = xs + sh.n1 * sh.n2;
-    T *const a1s = ts + sh.n1 * sh.m2;  // A1 (m1 x n1, ld m1) when fac_lds
+    T *const a1s = ts + sh.n1 * sh.m2;  // A1 (m1 x n1, ld m1)
     T *const a2s = a1s + sh.m1 * sh.n1; // A2 (m2 x n2, ld m2)
-    const int P = 64 / sh.m2;  // row groups
+    const int P = tpe / sh.m2;  // row groups
     const int p = lane / sh.m2, k2 = lane % sh.m2;
     const bool act = p < P;
     const int nx = sh.n1 * sh.n2;
-    for (long long g = (long long)blockIdx.x * 4; g < batch; g += (long long)gridDim.x * 4) {
-        const long long b = g + wid;  // this wave's entry; idle past the end
+    const int per = sh.per;
+    for (long long g = (long long)blockIdx.x * per; g < batch; g += (long long)gridDim.x * per) {
+        const long long b = g + wid;  // this slice's entry; idle past the end
         const bool live = b < batch;
         if (live) {
             const auto x = to_global(ent.x((int)b));
-            for (int w = lane; w < nx; w += 64) xs[w] = x[w];
-            if (sh.fac_lds) {
-                // both factors, compact (ld = rows), with coalesced column reads
-                const auto A1 = to_global(ent.factor((int)b, 0));
-                const auto A2 = to_global(ent.factor((int)b, 1));
-                for (int w = lane; w < sh.m1 * sh.n1; w += 64)
-                    a1s[w] = A1[(w % sh.m1) + (w / sh.m1) * sh.ld1];
-                for (int w = lane; w < sh.m2 * sh.n2; w += 64)
-                    a2s[w] = A2[(w % sh.m2) + (w / sh.m2) * sh.ld2];
-            }
+            for (int w = lane; w < nx; w += tpe) xs[w] = x[w];
+            // both factors, compact (ld = rows), with coalesced column reads
+            const auto A1 = to_global(ent.factor((int)b, 0));
+            const auto A2 = to_global(ent.factor((int)b, 1));
+            for (int w = lane; w < sh.m1 * sh.n1; w += tpe)
+                a1s[w] = A1[(w % sh.m1) + (w / sh.m1) * sh.ld1];
+            for (int w = lane; w < sh.m2 * sh.n2; w += tpe)
+                a2s[w] = A2[(w % sh.m2) + (w / sh.m2) * sh.ld2];
         }
         __syncthreads();
         if (live && act) {
-            const auto A2 = to_global(ent.factor((int)b, 1));
             for (int j1 = p; j1 < sh.n1; j1 += P) {
                 T acc = 0;
-                if (sh.fac_lds) {
 #pragma unroll 4
-                    for (int j2 = 0; j2 < sh.n2; ++j2)
-                        acc = __builtin_fma(a2s[k2 + j2 * sh.m2], xs[j1 * sh.n2 + j2], acc);
-                } else {
-#pragma unroll 4
-                    for (int j2 = 0; j2 < sh.n2; ++j2)
-                        acc = __builtin_fma(A2[k2 + j2 * sh.ld2], xs[j1 * sh.n2 + j2], acc);
-                }
+                for (int j2 = 0; j2 < sh.n2; ++j2)
+                    acc = __builtin_fma(a2s[k2 + j2 * sh.m2], xs[j1 * sh.n2 + j2], acc);
                 ts[j1 * sh.m2 + k2] = acc;
             }
         }
         __syncthreads();
         if (live && act) {
-            const auto A1 = to_global(ent.factor((int)b, 0));
             const auto y = to_global(ent.y((int)b));
             for (int k1 = p; k1 < sh.m1; k1 += P) {
                 T acc = 0;
-                if (sh.fac_lds) {
 #pragma unroll 4
-                    for (int j1 = 0; j1 < sh.n1; ++j1)
-                        acc = __builtin_fma(a1s[k1 + j1 * sh.m1], ts[j1 * sh.m2 + k2], acc);
-                } else {
-#pragma unroll 4
-                    for (int j1 = 0; j1 < sh.n1; ++j1)
-                        acc = __builtin_fma(A1[k1 + j1 * sh.ld1], ts[j1 * sh.m2 + k2], acc);
-                }
+                for (int j1 = 0; j1 < sh.n1; ++j1)
+                    acc = __builtin_fma(a1s[k1 + j1 * sh.m1], ts[j1 * sh.m2 + k2], acc);
                 y_accumulate<SHARED, T, T>(y, (long long)k1 * sh.m2 + k2, acc);
             }
         }
@@ -99,7 +90,7 @@ template <typename T, Layout LAY, bool SHARED>
 static kron_status d2_launch(const Entries<T, LAY> &e, const D2Shape &sh, int batch,
                              hipStream_t s)
 {
-    const size_t lds = (size_t)4 * sh.slice * sizeof(T);
+    const size_t lds = (size_t)sh.per * sh.slice * sizeof(T);
     const void *kern = reinterpret_cast<const void *>(kron_d2_kernel<T, LAY, SHARED>);
     if (lds > 65536) {
         hipError_t err = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -110,7 +101,7 @@ static kron_status d2_launch(const Entries<T, LAY> &e, const D2Shape &sh, int ba
         }
     }
     const long long resident = persistent_blocks(kern, lds);
-    const long long need = ((long long)batch + 3) / 4;
+    const long long need = ((long long)batch + sh.per - 1) / sh.per;
     const long long grid = std::max(1LL, std::min(need, resident > 0 ? resident : 2048LL));
     hipLaunchKernelGGL((kron_d2_kernel<T, LAY, SHARED>), dim3((unsigned)grid), dim3(256), lds, s,
                        e, sh, (long long)batch);
@@ -136,13 +127,13 @@ kron_status launch_d2(const KronArgs<T> &a)
     sh.ld1 = a.lda > 0 ? a.lda : sh.m1;
     sh.ld2 = a.lda > 0 ? a.lda : sh.m2;
     if (sh.ld1 < sh.m1 || sh.ld2 < sh.m2) return KRON_EINVAL;
-    // the factors join the slice while four slices stay within 32 KB (five
-    // workgroups per CU); above that they are read from global memory
+    // four slices (a wave per entry) with the factors while they stay within
+    // 32 KB (five workgroups per CU); above that one slice per workgroup,
+    // factors included (at most 4 x 32 x 32 elements: 32 KB in FP64)
     const int base = sh.n1 * sh.n2 + sh.n1 * sh.m2;
     const int with_fac = base + sh.m1 * sh.n1 + sh.m2 * sh.n2;
-    sh.fac_lds = (size_t)4 * with_fac * sizeof(T) <= 32768;
-    sh.slice = sh.fac_lds ? with_fac : base;
-    sh.slice = (sh.slice + 1) & ~1;  // 16-byte aligned slices for doubles
+    sh.per = (size_t)4 * with_fac * sizeof(T) <= 32768 ? 4 : 1;
+    sh.slice = (with_fac + 1) & ~1;  // 16-byte aligned slices for doubles
     if (a.batch == 0) return KRON_OK;
     if (a.layout == Layout::Batched) {
         // batched layout: square factors, n x n each, entries n^2 apart

@@ -355,8 +355,11 @@ def test_batched_d2_wave(n):
     assert np.array_equal(_gpu_batched(2, n, A, X, Y0), _oracle_batched(2, n, A, X, Y0))
 
 
-@pytest.mark.parametrize("n,lda,batch", [(12, 12, 3000), (10, 13, 700), (32, 32, 300)])
+@pytest.mark.parametrize("n,lda,batch", [(12, 12, 3000), (10, 13, 700), (32, 32, 300),
+                                         (16, 16, 500), (17, 17, 400), (24, 26, 300)])
 def test_xbatched_d2_wave_shared_y(n, lda, batch):
+    """n <= 16: a wave per entry (four slices of 32 KB at n = 16); from n = 17
+    a workgroup per entry with its factors in LDS."""
     ny = 5
     A_pool, X_pool, x_map, y_map = _xbatched_case(2, n, batch, 40, ny, lda, 19, kind="int")
     Y0 = np.zeros((ny, n * n))
