@@ -24,5 +24,11 @@ bool mfma_supported(int d, int n, bool *needs_w);
 template <typename T>
 kron_status launch_mfma(int d, int n, const Entries<T, Layout::Batched> &e, T *W, int batch,
                         hipStream_t s);
+// The pointer layout on the matrix cores: single-pass shapes (d, n) in
+// {(2, 16), (2, 32), (3, 16)}, factor leading dimension lda, Y updated with
+// atomics when entries may share it; KRON_ENOTSUP for other shapes.
+template <typename T>
+kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e, int lda,
+                            bool shared, int batch, hipStream_t s);
 
 }  // namespace kron

@@ -588,6 +588,13 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = launch_mfma<T>(a.d, a.n[0], e, a.W, a.batch, a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
+    // pointer layout, single-pass n = 16 / 32 shapes: also on the matrix cores
+    if (square && !a.trans && a.layout == Layout::Ptr) {
+        Entries<T, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
+        kron_status st = launch_mfma_ptr<T>(a.d, a.n[0], e, a.lda > 0 ? a.lda : a.n[0],
+                                            a.y_shared, a.batch, a.stream);
+        if (st != KRON_ENOTSUP) return st;
+    }
     // d = 1: a batched matrix-vector product, one lane per output row
     if (a.d == 1 && !a.trans) return launch_gemv(a);
     // d = 2, extents up to 32: a wave per entry

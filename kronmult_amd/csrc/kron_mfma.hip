@@ -133,11 +133,14 @@ struct MfmaGeom {
     }
 };
 
-// pass 0: whole entries (X -> Y); 1: fast modes (X -> W); 2: slow modes (W -> Y)
-template <typename T, int D, int N, int PASS>
+// pass 0: whole entries (X -> Y); 1: fast modes (X -> W); 2: slow modes (W -> Y).
+// The pointer layout (LAY = Ptr) runs pass 0 only; SHARED: entries may share
+// Y, which is then updated with device atomics.
+template <typename T, int D, int N, int PASS, Layout LAY = Layout::Batched, bool SHARED = false>
 __global__ void __launch_bounds__(256, 2)
-kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
+kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
 {
+    static_assert(LAY == Layout::Batched || PASS == 0, "pointer layout: single pass only");
     using G = MfmaGeom<T, D, N>;
     constexpr int TOT = PASS == 2 ? G::TOT2 : G::TOT1;
     constexpr int PER = TOT / 256;  // elements per thread
@@ -193,7 +196,7 @@ kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
         for_modes_desc<K1 - 1, K0>([&](auto kc) {
             constexpr int KM = decltype(kc)::value;
             constexpr int INNER = PASS == 2 ? cpow(N, G::H - 1 - KM) * G::C : cpow(N, D - 1 - KM);
-            mfma_afrag<T, N>(af, to_global(ent.factor(b, KM)), N, lane);
+            mfma_afrag<T, N>(af, to_global(ent.factor(b, KM)), lda, lane);
             mfma_mode<T, N, TOT, INNER>(tile, af, wave, lane);
             __syncthreads();
         });
@@ -203,6 +206,13 @@ kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
             for (int q = 0; q < PER; ++q) {
                 const int t = tid + q * 256;
                 w[gofs(k, t)] = tile[mfma_pos<T, N>(t)];
+            }
+        } else if constexpr (SHARED) {
+            const auto y = to_global(ent.y(b));
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int t = tid + q * 256;
+                atomic_add(y + gofs(k, t), tile[mfma_pos<T, N>(t)]);
             }
         } else {
             // Y read-modify-write in chunks of 8 (the next tile's prefetch
@@ -225,13 +235,14 @@ kron_mfma_kernel(Entries<T, Layout::Batched> ent, T *W, int batch)
     }
 }
 
-template <typename T, int D, int N, int PASS>
-static kron_status mfma_launch(const Entries<T, Layout::Batched> &e, T *W, int batch,
+template <typename T, int D, int N, int PASS, Layout LAY = Layout::Batched, bool SHARED = false>
+static kron_status mfma_launch(const Entries<T, LAY> &e, T *W, int lda, int batch,
                                hipStream_t s)
 {
     using G = MfmaGeom<T, D, N>;
     const size_t lds = G::lds_bytes();
-    const void *kern = reinterpret_cast<const void *>(kron_mfma_kernel<T, D, N, PASS>);
+    const void *kern =
+        reinterpret_cast<const void *>(kron_mfma_kernel<T, D, N, PASS, LAY, SHARED>);
     if (lds > 65536) {
         hipError_t err = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds);
@@ -243,8 +254,8 @@ static kron_status mfma_launch(const Entries<T, Layout::Batched> &e, T *W, int b
     const long long items = (long long)batch * G::tiles(PASS == 2 ? 2 : 1);
     const long long resident = persistent_blocks(kern, lds);
     const long long grid = std::max(1LL, std::min(items, resident > 0 ? resident : 1024LL));
-    hipLaunchKernelGGL((kron_mfma_kernel<T, D, N, PASS>), dim3((unsigned)grid), dim3(256), lds, s,
-                       e, W, batch);
+    hipLaunchKernelGGL((kron_mfma_kernel<T, D, N, PASS, LAY, SHARED>), dim3((unsigned)grid),
+                       dim3(256), lds, s, e, W, lda, batch);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error((int)err);
@@ -261,12 +272,26 @@ static kron_status mfma_run(const Entries<T, Layout::Batched> &e, T *W, int batc
     if constexpr (!G::OK) {
         return KRON_ENOTSUP;
     } else if constexpr (G::ONE) {
-        return mfma_launch<T, D, N, 0>(e, W, batch, s);
+        return mfma_launch<T, D, N, 0>(e, W, N, batch, s);
     } else {
         if (!W) return KRON_EWORKSPACE;
-        kron_status st = mfma_launch<T, D, N, 1>(e, W, batch, s);
+        kron_status st = mfma_launch<T, D, N, 1>(e, W, N, batch, s);
         if (st != KRON_OK) return st;
-        return mfma_launch<T, D, N, 2>(e, W, batch, s);
+        return mfma_launch<T, D, N, 2>(e, W, N, batch, s);
+    }
+}
+
+// Pointer layout: the single-pass shapes only (n^d <= kMfmaTile: no scratch).
+template <typename T, int D, int N>
+static kron_status mfma_run_ptr(const Entries<T, Layout::Ptr> &e, int lda, bool shared,
+                                int batch, hipStream_t s)
+{
+    using G = MfmaGeom<T, D, N>;
+    if constexpr (!G::ONE) {
+        return KRON_ENOTSUP;
+    } else {
+        if (shared) return mfma_launch<T, D, N, 0, Layout::Ptr, true>(e, nullptr, lda, batch, s);
+        return mfma_launch<T, D, N, 0, Layout::Ptr, false>(e, nullptr, lda, batch, s);
     }
 }
 
@@ -299,9 +324,26 @@ kron_status launch_mfma(int d, int n, const Entries<T, Layout::Batched> &e, T *W
     return KRON_ENOTSUP;
 }
 
+template <typename T>
+kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e, int lda,
+                            bool shared, int batch, hipStream_t s)
+{
+    if (batch == 0) return KRON_OK;
+#define KRON_MFMA_PCASE(DD, NN) \
+    if (d == DD && n == NN) return mfma_run_ptr<T, DD, NN>(e, lda, shared, batch, s);
+    KRON_MFMA_PCASE(2, 16) KRON_MFMA_PCASE(3, 16) KRON_MFMA_PCASE(2, 32)
+#undef KRON_MFMA_PCASE
+    return KRON_ENOTSUP;
+}
+
 template kron_status launch_mfma<double>(int, int, const Entries<double, Layout::Batched> &,
                                          double *, int, hipStream_t);
 template kron_status launch_mfma<float>(int, int, const Entries<float, Layout::Batched> &,
                                         float *, int, hipStream_t);
+
+template kron_status launch_mfma_ptr<double>(int, int, const Entries<double, Layout::Ptr> &,
+                                             int, bool, int, hipStream_t);
+template kron_status launch_mfma_ptr<float>(int, int, const Entries<float, Layout::Ptr> &, int,
+                                            bool, int, hipStream_t);
 
 }  // namespace kron

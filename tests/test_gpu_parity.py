@@ -299,6 +299,34 @@ def test_vbatched_d1_rectangular(m, n):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+# --- pointer layout on the matrix cores: single-pass n = 16 / 32 shapes --------
+
+@pytest.mark.parametrize("d,n,lda,batch,ny", [(3, 16, 18, 300, 7), (2, 32, 33, 200, 5),
+                                              (2, 16, 16, 600, 600), (3, 16, 16, 5, 1)])
+def test_xbatched_mfma_pointer_layout(d, n, lda, batch, ny):
+    """Padded leading dimensions, Y shared by many entries (atomics) and
+    nearly one Y per entry; integer values, so the comparison is exact."""
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 20, ny, lda, 29, kind="int")
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("d,n", [(3, 16), (2, 32)])
+def test_xbatched_mfma_pointer_layout_f32(d, n):
+    batch, ny = 64, 9
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 16, ny, n, 31)
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, n, A_pool.astype(np.float32), X_pool.astype(np.float32),
+                        x_map, Y0.astype(np.float32), y_map)
+    want = _oracle_xbatched(d, n, n, A_pool.astype(np.float32), X_pool.astype(np.float32),
+                            x_map, Y0, y_map)
+    # d n-term FP32 dot products per entry plus the atomic sums of up to
+    # batch / ny entries: k = d*n + batch/ny roundings of 2^-24 (x8)
+    assert H.rel_err(got, want) < 8 * (d * n + batch // ny + 1) * 2.0 ** -24
+
+
 # --- square n = 9..15 at d >= 3: the generic kernel's row-block contraction --
 
 @pytest.mark.parametrize("d,n", [(3, n) for n in range(9, 16)] +
@@ -646,7 +674,8 @@ def test_xbatched_plan(d, n, batch, ny):
 def test_pbatched_shared_y_integer_bit_exact(d, n):
     """kronmult{d}_pbatched: contiguous A, pointer X/Y/W, Y shared between
     entries (atomic last step), integer inputs, against the oracle; X must not
-    be written; (6, 6) and (3, 16) go through the per-entry W vectors."""
+    be written; (6, 6) goes through the per-entry W vectors, (3, 16) through
+    the matrix cores."""
     batch, nx, ny = 40 if n ** d < 20000 else 6, 9, 5
     E = n ** d
     A = H.integers(batch * d * n * n, 31 + d, -2, 2).reshape(batch, d, n * n)
