@@ -178,6 +178,29 @@ __device__ __forceinline__ void y_accumulate(P y, long long idx, Acc v)
     }
 }
 
+// A caller's scratch (W) as the kernels address it: the vector of entry b,
+// which runs in launch-group slot `slot` (see the Scratch kinds).
+template <typename T>
+struct ScratchDesc {
+    Scratch kind;
+    T *W;                 // BatchedW base
+    long long wstride;    // BatchedW elements per entry
+    T *const *pW;         // PtrSlots / PtrSlot0 / PtrEntry
+    T *Wbase;             // Linear
+    long long slot_stride;
+    __device__ __forceinline__ T *get(int slot, int b) const
+    {
+        switch (kind) {
+        case Scratch::BatchedW: return W + (long long)b * wstride;
+        case Scratch::PtrSlots: return pW[2 * slot];
+        case Scratch::PtrSlot0: return pW[0];
+        case Scratch::Linear: return Wbase + (long long)slot * slot_stride;
+        case Scratch::PtrEntry: return pW[b];
+        default: return nullptr;
+        }
+    }
+};
+
 // Quotient of small non-negative integers through the FP32 reciprocal:
 // int((x + 1/2) * rcp(d)) == x / d for 0 <= x < 2^21.  (x + 1/2) / d lies at
 // least 1/(2d) from every integer, and the rounding of rcp (1 ulp) and of the

@@ -24,11 +24,13 @@ bool mfma_supported(int d, int n, bool *needs_w);
 template <typename T>
 kron_status launch_mfma(int d, int n, const Entries<T, Layout::Batched> &e, T *W, int batch,
                         hipStream_t s);
-// The pointer layout on the matrix cores: single-pass shapes (d, n) in
-// {(2, 16), (2, 32), (3, 16)}, factor leading dimension lda, Y updated with
-// atomics when entries may share it; KRON_ENOTSUP for other shapes.
+// The pointer layouts on the matrix cores, the same (d, n) set: factor
+// leading dimension lda, Y updated with atomics.  Two-pass shapes need the
+// caller's scratch (scr, entries per launch group); KRON_ENOTSUP for shapes
+// outside the set.
 template <typename T>
-kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e, int lda,
-                            bool shared, int batch, hipStream_t s);
+kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e,
+                            const ScratchDesc<T> *scr, int group, int lda, int batch,
+                            hipStream_t s);
 
 }  // namespace kron

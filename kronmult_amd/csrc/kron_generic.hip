@@ -43,26 +43,6 @@ struct GenShape {
     long long nlo_out;      // prod_{k>=h} m_k
 };
 
-template <typename T>
-struct ScratchDesc {
-    Scratch kind;
-    T *W;                 // BatchedW base
-    long long wstride;    // BatchedW elements per entry
-    T *const *pW;         // PtrSlots / PtrSlot0 / PtrEntry
-    T *Wbase;             // Linear
-    long long slot_stride;
-    __device__ __forceinline__ T *get(int slot, int b) const
-    {
-        switch (kind) {
-        case Scratch::BatchedW: return W + (long long)b * wstride;
-        case Scratch::PtrSlots: return pW[2 * slot];
-        case Scratch::PtrSlot0: return pW[0];
-        case Scratch::Linear: return Wbase + (long long)slot * slot_stride;
-        case Scratch::PtrEntry: return pW[b];
-        default: return nullptr;
-        }
-    }
-};
 
 // Elements of one work item's input held per thread while the previous item
 // is contracted (the item's LDS chunk is at most kStagePer * 256 elements).
@@ -563,6 +543,45 @@ bool kron_groupable(const KronArgs<T> &a)
     return fast_groupable_rt<T>(a.d, N);
 }
 
+// The caller's scratch as the kernels address it (ScratchDesc), and how many
+// consecutive entries may run at once with distinct scratch vectors (group):
+// W(:,b) in the batched layout (wstride elements apart), pW[b] in pbatched,
+// pW[2*slot] / W + slot*stride over subbatchCount slots in xbatched /
+// vbatched.
+template <typename T>
+static kron_status scratch_of(const KronArgs<T> &a, long long wstride, ScratchDesc<T> &scr,
+                              int &group)
+{
+    scr = ScratchDesc<T>{a.scratch, nullptr, 0, nullptr, nullptr, 0};
+    group = a.batch;
+    switch (a.scratch) {
+    case Scratch::BatchedW:
+        if (!a.W) return KRON_EWORKSPACE;
+        scr.W = a.W;
+        scr.wstride = wstride;
+        break;
+    case Scratch::PtrSlots:
+    case Scratch::PtrSlot0:
+        if (!a.pW) return KRON_EWORKSPACE;
+        scr.pW = a.pW;
+        group = std::min(a.batch, a.scratch == Scratch::PtrSlot0 ? 1 : a.nslots);
+        break;
+    case Scratch::PtrEntry:
+        if (!a.pW) return KRON_EWORKSPACE;
+        scr.pW = a.pW;
+        break;
+    case Scratch::Linear:
+        if (!a.Wbase || a.nslots < 1) return KRON_EWORKSPACE;
+        scr.Wbase = a.Wbase;
+        scr.slot_stride = (long long)a.slot_stride;
+        group = std::min(a.batch, a.nslots);
+        break;
+    default:
+        return KRON_EWORKSPACE;
+    }
+    return group < 1 ? KRON_EWORKSPACE : KRON_OK;
+}
+
 template <typename T>
 kron_status launch_kron(const KronArgs<T> &a)
 {
@@ -588,11 +607,20 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = launch_mfma<T>(a.d, a.n[0], e, a.W, a.batch, a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
-    // pointer layout, single-pass n = 16 / 32 shapes: also on the matrix cores
-    if (square && !a.trans && a.layout == Layout::Ptr) {
+    // pointer layouts, n = 16 / 32: also on the matrix cores (two-pass shapes
+    // through the caller's scratch, group by group)
+    if (square && !a.trans && a.layout == Layout::Ptr && mfma_supported(a.d, a.n[0], nullptr)) {
         Entries<T, Layout::Ptr> e{a.pA, a.pX, a.pY, a.d};
-        kron_status st = launch_mfma_ptr<T>(a.d, a.n[0], e, a.lda > 0 ? a.lda : a.n[0],
-                                            a.y_shared, a.batch, a.stream);
+        bool needs_w = false;
+        mfma_supported(a.d, a.n[0], &needs_w);
+        ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
+        int group = a.batch;
+        if (needs_w) {
+            const kron_status st = scratch_of(a, 0, scr, group);
+            if (st != KRON_OK) return st;
+        }
+        kron_status st = launch_mfma_ptr<T>(a.d, a.n[0], e, needs_w ? &scr : nullptr, group,
+                                            a.lda > 0 ? a.lda : a.n[0], a.batch, a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
     // d = 1: a batched matrix-vector product, one lane per output row
@@ -607,33 +635,9 @@ kron_status launch_kron(const KronArgs<T> &a)
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
     int grid = a.batch;  // entries whose scratch vectors are distinct (one launch group)
     if (sh.h > 0) {
-        scr.kind = a.scratch;
-        switch (a.scratch) {
-        case Scratch::BatchedW:
-            if (!a.W) return KRON_EWORKSPACE;
-            scr.W = a.W;
-            scr.wstride = sh.nlo_in * sh.nhi_in;  // batched is square: sizeX
-            break;
-        case Scratch::PtrSlots:
-        case Scratch::PtrSlot0:
-            if (!a.pW) return KRON_EWORKSPACE;
-            scr.pW = a.pW;
-            grid = std::min(a.batch, a.scratch == Scratch::PtrSlot0 ? 1 : a.nslots);
-            break;
-        case Scratch::PtrEntry:
-            if (!a.pW) return KRON_EWORKSPACE;
-            scr.pW = a.pW;
-            break;
-        case Scratch::Linear:
-            if (!a.Wbase || a.nslots < 1) return KRON_EWORKSPACE;
-            scr.Wbase = a.Wbase;
-            scr.slot_stride = (long long)a.slot_stride;
-            grid = std::min(a.batch, a.nslots);
-            break;
-        default:
-            return KRON_EWORKSPACE;
-        }
-        if (grid < 1) return KRON_EWORKSPACE;
+        // batched is square here: W(:,b) holds sizeX elements
+        const kron_status st = scratch_of(a, sh.nlo_in * sh.nhi_in, scr, grid);
+        if (st != KRON_OK) return st;
     }
     if (a.layout == Layout::Batched) {
         Entries<T, Layout::Batched> e{a.A, a.X, a.Y, a.d, 0, 0, 0};

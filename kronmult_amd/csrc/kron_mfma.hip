@@ -133,14 +133,13 @@ struct MfmaGeom {
     }
 };
 
-// pass 0: whole entries (X -> Y); 1: fast modes (X -> W); 2: slow modes (W -> Y).
-// The pointer layout (LAY = Ptr) runs pass 0 only; SHARED: entries may share
-// Y, which is then updated with device atomics.
+// pass 0: whole entries (X -> Y); 1: fast modes (X -> W); 2: slow modes (W -> Y)
+// over the entries [b0, b1), entry b's scratch vector scr.get(b - b0, b).
+// SHARED: entries may share Y, which is then updated with device atomics.
 template <typename T, int D, int N, int PASS, Layout LAY = Layout::Batched, bool SHARED = false>
 __global__ void __launch_bounds__(256, 2)
-kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
+kron_mfma_kernel(Entries<T, LAY> ent, ScratchDesc<T> scr, int lda, int b0, int b1)
 {
-    static_assert(LAY == Layout::Batched || PASS == 0, "pointer layout: single pass only");
     using G = MfmaGeom<T, D, N>;
     constexpr int TOT = PASS == 2 ? G::TOT2 : G::TOT1;
     constexpr int PER = TOT / 256;  // elements per thread
@@ -149,7 +148,7 @@ kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
     extern __shared__ __align__(16) unsigned char kron_smem[];
     T *tile = reinterpret_cast<T *>(kron_smem);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long long items = (long long)batch * NT;
+    const long long items = (long long)(b1 - b0) * NT;
     const long long E = G::E;
 
     // HBM offset (within the entry) of tile element t of tile k
@@ -161,7 +160,7 @@ kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
     };
     auto src_of = [&](int b) -> gptr<const T> {
         if constexpr (PASS == 2)
-            return to_global((const T *)W) + (long long)b * E;
+            return to_global((const T *)scr.get(b - b0, b));
         else
             return to_global(ent.x(b));
     };
@@ -170,7 +169,7 @@ kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
     // 1% faster at d=3 n=32 and d=4 n=16.
     auto entry_of = [&](long long it) -> int {
         const int q = (int)(it / NT);
-        return PASS == 2 ? batch - 1 - q : q;
+        return PASS == 2 ? b1 - 1 - q : b0 + q;
     };
     T v[PER];
     auto fetch = [&](long long it) {
@@ -201,7 +200,7 @@ kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
             __syncthreads();
         });
         if constexpr (PASS == 1) {
-            const auto w = to_global(W) + (long long)b * E;
+            const auto w = to_global(scr.get(b - b0, b));
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int t = tid + q * 256;
@@ -236,8 +235,8 @@ kron_mfma_kernel(Entries<T, LAY> ent, T *W, int lda, int batch)
 }
 
 template <typename T, int D, int N, int PASS, Layout LAY = Layout::Batched, bool SHARED = false>
-static kron_status mfma_launch(const Entries<T, LAY> &e, T *W, int lda, int batch,
-                               hipStream_t s)
+static kron_status mfma_launch(const Entries<T, LAY> &e, const ScratchDesc<T> &scr, int lda,
+                               int b0, int b1, hipStream_t s)
 {
     using G = MfmaGeom<T, D, N>;
     const size_t lds = G::lds_bytes();
@@ -251,11 +250,11 @@ static kron_status mfma_launch(const Entries<T, LAY> &e, T *W, int lda, int batc
             return KRON_EHIP;
         }
     }
-    const long long items = (long long)batch * G::tiles(PASS == 2 ? 2 : 1);
+    const long long items = (long long)(b1 - b0) * G::tiles(PASS == 2 ? 2 : 1);
     const long long resident = persistent_blocks(kern, lds);
     const long long grid = std::max(1LL, std::min(items, resident > 0 ? resident : 1024LL));
     hipLaunchKernelGGL((kron_mfma_kernel<T, D, N, PASS, LAY, SHARED>), dim3((unsigned)grid),
-                       dim3(256), lds, s, e, W, lda, batch);
+                       dim3(256), lds, s, e, scr, lda, b0, b1);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error((int)err);
@@ -271,27 +270,42 @@ static kron_status mfma_run(const Entries<T, Layout::Batched> &e, T *W, int batc
     using G = MfmaGeom<T, D, N>;
     if constexpr (!G::OK) {
         return KRON_ENOTSUP;
-    } else if constexpr (G::ONE) {
-        return mfma_launch<T, D, N, 0>(e, W, N, batch, s);
     } else {
-        if (!W) return KRON_EWORKSPACE;
-        kron_status st = mfma_launch<T, D, N, 1>(e, W, N, batch, s);
-        if (st != KRON_OK) return st;
-        return mfma_launch<T, D, N, 2>(e, W, N, batch, s);
+        const ScratchDesc<T> scr{Scratch::BatchedW, W, G::E, nullptr, nullptr, 0};
+        if constexpr (G::ONE) {
+            return mfma_launch<T, D, N, 0>(e, scr, N, 0, batch, s);
+        } else {
+            if (!W) return KRON_EWORKSPACE;
+            kron_status st = mfma_launch<T, D, N, 1>(e, scr, N, 0, batch, s);
+            if (st != KRON_OK) return st;
+            return mfma_launch<T, D, N, 2>(e, scr, N, 0, batch, s);
+        }
     }
 }
 
-// Pointer layout: the single-pass shapes only (n^d <= kMfmaTile: no scratch).
+// Pointer layout, Y possibly shared (atomics).  Two-pass shapes run in groups
+// of `group` consecutive entries whose scratch vectors are distinct: pass 1
+// and pass 2 of one group, then the next group.
 template <typename T, int D, int N>
-static kron_status mfma_run_ptr(const Entries<T, Layout::Ptr> &e, int lda, bool shared,
-                                int batch, hipStream_t s)
+static kron_status mfma_run_ptr(const Entries<T, Layout::Ptr> &e, const ScratchDesc<T> *scr,
+                                int group, int lda, int batch, hipStream_t s)
 {
     using G = MfmaGeom<T, D, N>;
-    if constexpr (!G::ONE) {
+    if constexpr (!G::OK) {
         return KRON_ENOTSUP;
+    } else if constexpr (G::ONE) {
+        const ScratchDesc<T> none{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
+        return mfma_launch<T, D, N, 0, Layout::Ptr, true>(e, none, lda, 0, batch, s);
     } else {
-        if (shared) return mfma_launch<T, D, N, 0, Layout::Ptr, true>(e, nullptr, lda, batch, s);
-        return mfma_launch<T, D, N, 0, Layout::Ptr, false>(e, nullptr, lda, batch, s);
+        if (!scr || group < 1) return KRON_EWORKSPACE;
+        for (int b0 = 0; b0 < batch; b0 += group) {
+            const int b1 = std::min(batch, b0 + group);
+            kron_status st = mfma_launch<T, D, N, 1, Layout::Ptr, true>(e, *scr, lda, b0, b1, s);
+            if (st != KRON_OK) return st;
+            st = mfma_launch<T, D, N, 2, Layout::Ptr, true>(e, *scr, lda, b0, b1, s);
+            if (st != KRON_OK) return st;
+        }
+        return KRON_OK;
     }
 }
 
@@ -325,13 +339,14 @@ kron_status launch_mfma(int d, int n, const Entries<T, Layout::Batched> &e, T *W
 }
 
 template <typename T>
-kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e, int lda,
-                            bool shared, int batch, hipStream_t s)
+kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e, const ScratchDesc<T> *scr,
+                            int group, int lda, int batch, hipStream_t s)
 {
     if (batch == 0) return KRON_OK;
 #define KRON_MFMA_PCASE(DD, NN) \
-    if (d == DD && n == NN) return mfma_run_ptr<T, DD, NN>(e, lda, shared, batch, s);
-    KRON_MFMA_PCASE(2, 16) KRON_MFMA_PCASE(3, 16) KRON_MFMA_PCASE(2, 32)
+    if (d == DD && n == NN) return mfma_run_ptr<T, DD, NN>(e, scr, group, lda, batch, s);
+    KRON_MFMA_PCASE(2, 16) KRON_MFMA_PCASE(3, 16) KRON_MFMA_PCASE(4, 16) KRON_MFMA_PCASE(5, 16)
+    KRON_MFMA_PCASE(2, 32) KRON_MFMA_PCASE(3, 32)
 #undef KRON_MFMA_PCASE
     return KRON_ENOTSUP;
 }
@@ -342,8 +357,10 @@ template kron_status launch_mfma<float>(int, int, const Entries<float, Layout::B
                                         float *, int, hipStream_t);
 
 template kron_status launch_mfma_ptr<double>(int, int, const Entries<double, Layout::Ptr> &,
-                                             int, bool, int, hipStream_t);
-template kron_status launch_mfma_ptr<float>(int, int, const Entries<float, Layout::Ptr> &, int,
-                                            bool, int, hipStream_t);
+                                             const ScratchDesc<double> *, int, int, int,
+                                             hipStream_t);
+template kron_status launch_mfma_ptr<float>(int, int, const Entries<float, Layout::Ptr> &,
+                                            const ScratchDesc<float> *, int, int, int,
+                                            hipStream_t);
 
 }  // namespace kron

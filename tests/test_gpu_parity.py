@@ -313,6 +313,39 @@ def test_xbatched_mfma_pointer_layout(d, n, lda, batch, ny):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("d,n,lda,batch,ny,subbatch", [(4, 16, 16, 7, 3, 2), (3, 32, 34, 5, 2, 0),
+                                                       (5, 16, 16, 3, 2, 1), (4, 16, 17, 1, 1, 0)])
+def test_xbatched_mfma_two_pass(d, n, lda, batch, ny, subbatch):
+    """Two-pass n = 16 / 32 shapes in the pointer layout: the W slots
+    pW[2*slot] reused group by group (subbatchCount slots), and a single entry
+    (pW[0]); integer values, exact comparison."""
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 4, ny, lda, 37, kind="int")
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map, subbatch=subbatch)
+    want = _oracle_xbatched(d, n, lda, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("slots", [1, 2])
+def test_vbatched_square_mfma_two_pass(slots):
+    """Square vbatched d4n16 on the matrix cores through the W + slot*stride
+    scratch rule, slot by slot."""
+    m = n = [16] * 4
+    batch, ny = 5, 2
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, 16, ny)
+    _, sxw = _sizes(m, n)
+    Y0 = np.zeros((ny, 16 ** 4))
+    Yw = Y0.copy()
+    Wcap = slots * 2 * sxw * 8
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * 4).reshape(batch, 4), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.full((Wcap // 8,), float("nan"), dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * 4)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert H.rel_err(host(Yd), Yw) < F64_TOL
+
+
 @pytest.mark.parametrize("d,n", [(3, 16), (2, 32)])
 def test_xbatched_mfma_pointer_layout_f32(d, n):
     batch, ny = 64, 9
@@ -670,12 +703,12 @@ def test_xbatched_plan(d, n, batch, ny):
 # --- pbatched ------------------------------------------------------------------
 
 @pytest.mark.parametrize("d,n", [(1, 5), (2, 8), (3, 4), (4, 6), (5, 3), (6, 4), (6, 6),
-                                 (3, 16)])
+                                 (3, 16), (4, 16)])
 def test_pbatched_shared_y_integer_bit_exact(d, n):
     """kronmult{d}_pbatched: contiguous A, pointer X/Y/W, Y shared between
     entries (atomic last step), integer inputs, against the oracle; X must not
     be written; (6, 6) goes through the per-entry W vectors, (3, 16) through
-    the matrix cores."""
+    the matrix cores, (4, 16) through both."""
     batch, nx, ny = 40 if n ** d < 20000 else 6, 9, 5
     E = n ** d
     A = H.integers(batch * d * n * n, 31 + d, -2, 2).reshape(batch, d, n * n)
