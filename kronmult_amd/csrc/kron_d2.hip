@@ -47,9 +47,23 @@ kron_d2_kernel(Entries<T, LAY> ent, D2Shape sh, long long batch)
     const bool act = p < P;
     const int nx = sh.n1 * sh.n2;
     const int per = sh.per;
+    // private Y: a thread's first YPF outputs are read with the entry's other
+    // loads, so phase 2 does not wait on them
+    constexpr int YPF = SHARED ? 1 : 4;
     for (long long g = (long long)blockIdx.x * per; g < batch; g += (long long)gridDim.x * per) {
         const long long b = g + wid;  // this slice's entry; idle past the end
         const bool live = b < batch;
+        T yv[YPF];
+        if constexpr (!SHARED) {
+            if (live && act) {
+                const auto y = to_global(ent.y((int)b));
+#pragma unroll
+                for (int i = 0; i < YPF; ++i) {
+                    const int k1 = p + i * P;
+                    yv[i] = k1 < sh.m1 ? y[(long long)k1 * sh.m2 + k2] : T(0);
+                }
+            }
+        }
         if (live) {
             const auto x = to_global(ent.x((int)b));
             for (int w = lane; w < nx; w += tpe) xs[w] = x[w];
@@ -74,13 +88,21 @@ kron_d2_kernel(Entries<T, LAY> ent, D2Shape sh, long long batch)
         __syncthreads();
         if (live && act) {
             const auto y = to_global(ent.y((int)b));
-            for (int k1 = p; k1 < sh.m1; k1 += P) {
+            auto row = [&](int k1) __attribute__((always_inline)) {
                 T acc = 0;
 #pragma unroll 4
                 for (int j1 = 0; j1 < sh.n1; ++j1)
                     acc = __builtin_fma(a1s[k1 + j1 * sh.m1], ts[j1 * sh.m2 + k2], acc);
-                y_accumulate<SHARED, T, T>(y, (long long)k1 * sh.m2 + k2, acc);
+                return acc;
+            };
+            int k1 = p;
+            if constexpr (!SHARED) {
+#pragma unroll
+                for (int i = 0; i < YPF; ++i, k1 += P)
+                    if (k1 < sh.m1) y[(long long)k1 * sh.m2 + k2] = yv[i] + row(k1);
             }
+            for (; k1 < sh.m1; k1 += P)
+                y_accumulate<SHARED, T, T>(y, (long long)k1 * sh.m2 + k2, row(k1));
         }
         __syncthreads();  // the slices are rewritten by the next entries
     }
