@@ -424,6 +424,36 @@ def bench_large_n(args):
                       "cases": rows}), flush=True)
 
 
+def bench_fused_shapes(args):
+    """Batched FP64 shapes of the fused kernels, one JSON line each: d = 2..6,
+    n = 2..8, every n^d from 8 to 20,480 elements; 2^25 / n^d entries (at most
+    4M), uniform [0, 1) data; algorithmic bytes X + Y read-modify-write + A."""
+    import torch
+    import kronmult_amd as K
+    dev = "cuda:0"
+    stream = torch.cuda.current_stream()
+    torch.manual_seed(2024)
+    for d in range(2, 7):
+        for n in range(2, 9):
+            E = n ** d
+            if E > 20480 or E < 8:
+                continue
+            batch = max(64, min(1 << 22, (1 << 25) // E))
+            A = torch.rand(batch * d * n * n, dtype=torch.float64, device=dev)
+            X = torch.rand(batch * E, dtype=torch.float64, device=dev)
+            Y = torch.zeros(batch * E, dtype=torch.float64, device=dev)
+            W = torch.empty(batch * E, dtype=torch.float64, device=dev)
+            ms = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, W, batch, stream=stream),
+                             args.steps, args.warmup, stream)
+            alg = 8 * (3 * E + d * n * n) * batch
+            print(json.dumps({"workload": "kronmult_batched fused shape", "d": d, "n": n,
+                              "dtype": "f64", "batch": batch, "ms": round(ms, 4),
+                              "alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+                              "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}),
+                  flush=True)
+            del A, X, Y, W
+
+
 def bench_shapes(args):
     """Batched shapes beyond the fused kernels, one JSON line each: d = 1 from
     n = 8 (matrix-vector kernel), d = 2 at n = 9..32 (a wave per entry), d = 3
@@ -502,6 +532,8 @@ def main():
         bench_copy(args)
     elif args.workload == "shapes":
         bench_shapes(args)
+    elif args.workload == "fused_shapes":
+        bench_fused_shapes(args)
     else:
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
