@@ -16,9 +16,9 @@
 // other traffic stays in LDS.  The contraction order differs from the
 // reference's (1, 2, ..., D), which changes only the rounding.
 //
-// Small entries share a 256-thread workgroup: G = pow2ceil(N^(D-1)) threads
-// (at most 256) per entry, 256/G entries per workgroup; large ones (see
-// FastGeom) take a 1,024-thread workgroup each.
+// Small entries share a 256-thread workgroup: G = pow2ceil(N^(D-1)) or
+// N^(D-1) threads (at most 256) per entry, 256/G entries per workgroup; large
+// ones (see FastGeom) take a 1,024-thread workgroup each.
 #pragma once
 
 #include "kron_device.hpp"
@@ -36,7 +36,13 @@ struct FastGeom {
     static constexpr int F = E / N;              // fibers per mode
     static constexpr bool BIG = (size_t)E * sizeof(T) > 81920 && F >= 1024;
     static constexpr int THREADS = !BIG ? 256 : (size_t)N * N * sizeof(T) <= 200 ? 1024 : 512;
-    static constexpr int G = F >= THREADS ? THREADS : pow2ceil(F);  // threads per entry
+    // threads per entry: F rounded up to a power of two, or exactly F when
+    // that packs more entries into the workgroup (F < 128 only, so P stays
+    // above 1); threads from P * G on then sit idle (d3n6: 7 entries of 36
+    // threads instead of 4 of 64)
+    static constexpr int G2 = F >= THREADS ? THREADS : pow2ceil(F);
+    static constexpr bool EXACT = !BIG && F < 128 && THREADS / F > THREADS / G2;
+    static constexpr int G = EXACT ? F : G2;
     static constexpr int P = THREADS / G;        // entries per workgroup
     static constexpr size_t lds_bytes(size_t elem) { return (size_t)P * E * elem; }
 };
@@ -303,6 +309,9 @@ kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
     const int tid = threadIdx.x;
     const int slot = (P == 1) ? 0 : tid / G;
     const int g0 = (P == 1) ? tid : tid % G;
+    // threads past P * G (G not a power of two) own no entry: entry() gives
+    // them batch, which every work guard (b < batch) skips
+    const bool idle = P * G < Geo::THREADS && slot >= P;
     const int stride = gridDim.x * P;
     Acc a[N][N];
     static_assert(!GROUPED || fast_groupable<T, D, N>(), "grouping needs P == 1, D >= 2");
@@ -322,12 +331,12 @@ kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
         if constexpr (GROUPED)
             return to_const(ord.perm)[r];
         else
-            return r + slot;
+            return idle ? batch : r + slot;
     };
 
     if constexpr (D == 1) {
         const int g = g0;
-        for (int b = blockIdx.x * P + slot; b < batch; b += stride) {
+        for (int b = idle ? batch : blockIdx.x * P + slot; b < batch; b += stride) {
             load_factor<T, Acc, N, TRANS, UNI>(a, ent.template factor_g<UNI>(b, 0), lda);
             const auto x = ent.template x_g<UNI>(b);
             const auto y = ent.template y_g<UNI>(b);
