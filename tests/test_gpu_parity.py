@@ -299,6 +299,34 @@ def test_vbatched_d1_rectangular(m, n):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (m, n)
 
 
+# --- fused kernel with exactly n^(d-1) threads per entry (not a power of two) --
+
+@pytest.mark.parametrize("d,n,batch", [(3, 6, 501), (5, 3, 250), (2, 5, 1003), (3, 3, 29),
+                                       (2, 7, 37)])
+def test_xbatched_exact_thread_groups(d, n, batch):
+    """Several entries per workgroup with idle threads past the last one;
+    batches that leave a partial last workgroup; shared Y (atomics)."""
+    ny = 7
+    A_pool, X_pool, x_map, y_map = _xbatched_case(d, n, batch, 11, ny, n, 41, kind="int")
+    Y0 = np.zeros((ny, n ** d))
+    got = _gpu_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    want = _oracle_xbatched(d, n, n, A_pool, X_pool, x_map, Y0, y_map)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("d,n", [(3, 6), (5, 3)])
+def test_batched_exact_thread_groups_f32(d, n):
+    batch = 333
+    E = n ** d
+    A = H.uniform(batch * d * n * n, 121).reshape(batch, d, n * n)
+    X = H.uniform(batch * E, 122).reshape(batch, E)
+    Y0 = H.uniform(batch * E, 123).reshape(batch, E)
+    got = _gpu_batched(d, n, A.astype(np.float32), X.astype(np.float32), Y0.astype(np.float32))
+    want = _oracle_batched(d, n, A.astype(np.float32), X.astype(np.float32),
+                           Y0.astype(np.float32))
+    assert H.rel_err(got, want) < 8 * d * n * 2.0 ** -24
+
+
 # --- pointer layout on the matrix cores: single-pass n = 16 / 32 shapes --------
 
 @pytest.mark.parametrize("d,n,lda,batch,ny", [(3, 16, 18, 300, 7), (2, 32, 33, 200, 5),
