@@ -454,6 +454,41 @@ def bench_fused_shapes(args):
             del A, X, Y, W
 
 
+def bench_kgemm(args):
+    """kgemm_nn_batched / kgemm_nt_batched (FP64), one JSON line per case:
+    C_b = A_b op(B_b) + C_b (alpha = 1, beta = 1) over contiguous per-entry
+    operands, shapes of the recursion's steps at d = 6, n = 4 (mm = 1024,
+    nn = kk = 4) and a square 16^3 case; algorithmic bytes per entry
+    8*(mm*kk + kk*nn + 2*mm*nn)."""
+    import torch
+    import kronmult_amd as K
+    dev = "cuda:0"
+    stream = torch.cuda.current_stream()
+    torch.manual_seed(2024)
+    for tr, mm, nn, kk, batch in (("nt", 1024, 4, 4, 65536), ("nn", 1024, 4, 4, 65536),
+                                  ("nn", 16, 16, 16, 262144), ("nt", 16, 16, 16, 262144)):
+        A = torch.rand(batch * mm * kk, dtype=torch.float64, device=dev)
+        B = torch.rand(batch * kk * nn, dtype=torch.float64, device=dev)
+        C = torch.zeros(batch * mm * nn, dtype=torch.float64, device=dev)
+        idx = torch.arange(batch, dtype=torch.int64, device=dev)
+        pA = A.data_ptr() + idx * (mm * kk * 8)
+        pB = B.data_ptr() + idx * (kk * nn * 8)
+        pC = C.data_ptr() + idx * (mm * nn * 8)
+        ldA = torch.full((batch,), mm, dtype=torch.int32, device=dev)
+        ldB = torch.full((batch,), kk if tr == "nn" else nn, dtype=torch.int32, device=dev)
+        ldC = torch.full((batch,), mm, dtype=torch.int32, device=dev)
+        f = K.kgemm_nn_batched if tr == "nn" else K.kgemm_nt_batched
+        ms = _time_steps(lambda: f(mm, nn, kk, 1.0, pA, ldA, pB, ldB, 1.0, pC, ldC, batch,
+                                   stream=stream), args.steps, args.warmup, stream)
+        alg = 8 * (mm * kk + kk * nn + 2 * mm * nn) * batch
+        print(json.dumps({"workload": "kgemm_%s_batched" % tr, "mm": mm, "nn": nn, "kk": kk,
+                          "dtype": "f64", "batch": batch, "ms": round(ms, 4),
+                          "alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+                          "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}),
+              flush=True)
+        del A, B, C
+
+
 def bench_shapes(args):
     """Batched shapes beyond the fused kernels, one JSON line each: d = 1 from
     n = 8 (matrix-vector kernel), d = 2 at n = 9..32 (a wave per entry), d = 3
@@ -534,6 +569,8 @@ def main():
         bench_shapes(args)
     elif args.workload == "fused_shapes":
         bench_fused_shapes(args)
+    elif args.workload == "kgemm":
+        bench_kgemm(args)
     else:
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
