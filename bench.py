@@ -458,7 +458,7 @@ def bench_kgemm(args):
     """kgemm_nn_batched / kgemm_nt_batched (FP64), one JSON line per case:
     C_b = A_b op(B_b) + C_b (alpha = 1, beta = 1) over contiguous per-entry
     operands, shapes of the recursion's steps at d = 6, n = 4 (mm = 1024,
-    nn = kk = 4) and a square 16^3 case; algorithmic bytes per entry
+    nn = kk = 4), a square 16^3 case and 4^3 entries; algorithmic bytes per entry
     8*(mm*kk + kk*nn + 2*mm*nn)."""
     import torch
     import kronmult_amd as K
@@ -466,7 +466,8 @@ def bench_kgemm(args):
     stream = torch.cuda.current_stream()
     torch.manual_seed(2024)
     for tr, mm, nn, kk, batch in (("nt", 1024, 4, 4, 65536), ("nn", 1024, 4, 4, 65536),
-                                  ("nn", 16, 16, 16, 262144), ("nt", 16, 16, 16, 262144)):
+                                  ("nn", 16, 16, 16, 262144), ("nt", 16, 16, 16, 262144),
+                                  ("nn", 4, 4, 4, 1 << 20)):
         A = torch.rand(batch * mm * kk, dtype=torch.float64, device=dev)
         B = torch.rand(batch * kk * nn, dtype=torch.float64, device=dev)
         C = torch.zeros(batch * mm * nn, dtype=torch.float64, device=dev)

@@ -5,7 +5,8 @@
 // kgemm_nt_batched.hpp:8-75) and their device GEMMs (kgemm_nn.hpp:227-406,
 // kgemm_nt.hpp:215-385).  Each output element is one work item; consecutive
 // threads take consecutive rows i, so loads of A(i,k) and the store of C(i,j)
-// are coalesced and B(k,j) (or B(j,k)) is a broadcast.  Accumulation is in
+// are coalesced and B(k,j) (or B(j,k)) is a broadcast; outputs of at most
+// 128 elements pack several entries into a workgroup.  Accumulation is in
 // double for both element types (kgemm_nn.hpp:688-701, kgemm_nt.hpp:557-571),
 // summing k = 1..kk in order as the reference does.  Large outputs are split
 // over several workgroups per entry.
@@ -18,23 +19,37 @@
 
 namespace kron {
 
-template <typename T, bool TRANSB, int BETA_MODE>
+template <typename T, bool TRANSB, int BETA_MODE, bool PACKED>
 __global__ void __launch_bounds__(256)
 kgemm_batched_kernel(int mm, int nn, int kk, T alpha, const T *const *Aarray,
                      const int *ldAarray, const T *const *Barray, const int *ldBarray,
-                     T beta, T *const *Carray, const int *ldCarray, int tiles, int batch)
+                     T beta, T *const *Carray, const int *ldCarray, int tiles, int epw,
+                     int batch)
 {
-    const int b = blockIdx.x / tiles;
-    const int tile = blockIdx.x % tiles;
+    // PACKED: small outputs (mm*nn <= 128), epw entries per workgroup, each
+    // on mm*nn consecutive threads; otherwise `tiles` workgroups per entry
+    const long long total = (long long)mm * nn;
+    int b, tile, t0, nthr;
+    if constexpr (PACKED) {
+        b = blockIdx.x * epw + (int)(threadIdx.x / total);
+        tile = 0;
+        t0 = (int)(threadIdx.x % total);
+        nthr = (int)total;
+        if (threadIdx.x >= epw * total) return;
+    } else {
+        b = blockIdx.x / tiles;
+        tile = blockIdx.x % tiles;
+        t0 = threadIdx.x;
+        nthr = blockDim.x;
+    }
     if (b >= batch) return;
     const T *__restrict__ A = Aarray[b];
     const T *__restrict__ B = Barray[b];
     T *C = Carray[b];
     const int lda = ldAarray[b], ldb = ldBarray[b], ldc = ldCarray[b];
     const double dalpha = alpha, dbeta = beta;
-    const long long total = (long long)mm * nn;
-    const long long step = (long long)tiles * blockDim.x;
-    for (long long w = (long long)tile * blockDim.x + threadIdx.x; w < total; w += step) {
+    const long long step = (long long)tiles * nthr;
+    for (long long w = (long long)tile * nthr + t0; w < total; w += step) {
         const int i = (int)(w % mm);
         const int j = (int)(w / mm);
         double c = 0;
@@ -68,13 +83,24 @@ kron_status launch_kgemm(bool transB, int mm, int nn, int kk, T alpha, const T *
     long long tiles = (total + 1023) / 1024;
     if (tiles > 1024) tiles = 1024;
     if (tiles < 1) tiles = 1;
-    const long long grid = tiles * (long long)batch;
+    // outputs of at most 128 elements: several entries per workgroup
+    const int epw = total <= 128 ? (int)(256 / total) : 1;
+    const long long grid = epw > 1 ? ((long long)batch + epw - 1) / epw : tiles * (long long)batch;
     if (grid > 0x7fffffffLL) return KRON_EINVAL;
     const int mode = (beta == T(1)) ? 1 : (beta == T(0)) ? 0 : 2;
 #define KGEMM_LAUNCH(TB, MODE)                                                              \
-    hipLaunchKernelGGL((kgemm_batched_kernel<T, TB, MODE>), dim3((unsigned)grid), dim3(256), \
-                       0, stream, mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC,          \
-                       (int)tiles, batch)
+    do {                                                                                    \
+        if (epw > 1)                                                                        \
+            hipLaunchKernelGGL((kgemm_batched_kernel<T, TB, MODE, true>),                   \
+                               dim3((unsigned)grid), dim3(256), 0, stream, mm, nn, kk,      \
+                               alpha, A, ldA, B, ldB, beta, C, ldC, (int)tiles, epw,        \
+                               batch);                                                      \
+        else                                                                                \
+            hipLaunchKernelGGL((kgemm_batched_kernel<T, TB, MODE, false>),                  \
+                               dim3((unsigned)grid), dim3(256), 0, stream, mm, nn, kk,      \
+                               alpha, A, ldA, B, ldB, beta, C, ldC, (int)tiles, epw,        \
+                               batch);                                                      \
+    } while (0)
     if (transB) {
         if (mode == 1) KGEMM_LAUNCH(true, 1);
         else if (mode == 0) KGEMM_LAUNCH(true, 0);

@@ -589,6 +589,40 @@ def test_kgemm(trans, beta, dtype):
         assert H.rel_err(got, Cw) < tol, (mm, nn, kk)
 
 
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("beta", [0.0, 1.0, 1.2])
+@pytest.mark.parametrize("mm,nn,kk,batch", [(4, 4, 4, 1001), (5, 3, 7, 300), (1, 64, 3, 77),
+                                            (16, 8, 2, 9)])
+def test_kgemm_small_outputs_packed(mm, nn, kk, batch, trans, beta):
+    """Outputs of at most 128 elements: several entries per workgroup, a
+    partial last workgroup, padded leading dimensions (ldA = mm + 3,
+    ldC = mm + 1)."""
+    lda, ldc = mm + 3, mm + 1
+    ldb = (kk if trans == "N" else nn) + 2
+    bcols = nn if trans == "N" else kk
+    A = np.full((batch, lda * kk), np.nan)
+    B = np.full((batch, ldb * bcols), np.nan)
+    C0 = np.full((batch, ldc * nn), 0.0 if beta == 0.0 else 0.5)
+    for b in range(batch):
+        a = H.uniform(mm * kk, 7 * b + 1).reshape(kk, mm)
+        A[b].reshape(kk, lda)[:, :mm] = a
+        rows = kk if trans == "N" else nn
+        B[b].reshape(bcols, ldb)[:, :rows] = H.uniform(rows * bcols, 7 * b + 2).reshape(bcols, rows)
+    Cw = C0.copy()
+    oracle.kgemm(O, trans, mm, nn, kk, 1.3, np.nan_to_num(A), [lda] * batch, np.nan_to_num(B),
+                 [ldb] * batch, beta, Cw, [ldc] * batch, batch)
+    Ad, Bd, Cd = dev(A), dev(B), dev(C0)
+    f = K.kgemm_nn_batched if trans == "N" else K.kgemm_nt_batched
+    f(mm, nn, kk, 1.3, ptrs(Ad, np.arange(batch)), ints([lda] * batch),
+      ptrs(Bd, np.arange(batch)), ints([ldb] * batch), beta, ptrs(Cd, np.arange(batch)),
+      ints([ldc] * batch), batch)
+    got = host(Cd)
+    # the padding rows of C (i >= mm) are never written
+    pad = got.reshape(batch, nn, ldc)[:, :, mm:]
+    assert np.all(pad == (0.0 if beta == 0.0 else 0.5))
+    assert H.rel_err(got, Cw) < 1e-12, (mm, nn, kk)
+
+
 def test_kgemm_shared_c_atomic():
     """beta == 1 with every entry naming the same C: atomic accumulation."""
     mm, nn, kk, batch = 16, 16, 4, 200
