@@ -630,6 +630,11 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = launch_d2(a);
         if (st != KRON_ENOTSUP) return st;
     }
+    // d >= 3 with mixed per-mode extents (vbatched): a wave per small entry
+    if (!square && a.d >= 3 && !a.trans) {
+        kron_status st = launch_small(a);
+        if (st != KRON_ENOTSUP) return st;
+    }
     GenShape sh{};
     if (!make_generic_shape(a, sh)) return KRON_ENOTSUP;
     ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};

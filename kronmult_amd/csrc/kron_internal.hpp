@@ -74,6 +74,11 @@ kron_status launch_gemv(const KronArgs<T> &a);
 // other shapes or transposed factors.
 template <typename T>
 kron_status launch_d2(const KronArgs<T> &a);
+// d >= 3 entries with mixed per-mode extents whose intermediates fit a
+// wave's LDS slice (kron_small.hip): a wave per entry.  KRON_ENOTSUP when
+// the shape does not fit.
+template <typename T>
+kron_status launch_small(const KronArgs<T> &a);
 
 // Launch the forward kernel family.  Returns KRON_OK, KRON_ENOTSUP when no
 // kernel covers the shape, KRON_EWORKSPACE when scratch is needed but absent,

@@ -543,6 +543,59 @@ def test_vbatched_square_rowblocks(nn, d, slots):
     assert H.rel_err(host(Yd), Yw) < F64_TOL, (nn, d)
 
 
+@pytest.mark.parametrize("m,n", [([4, 5, 6], [4, 5, 6]), ([3, 2, 5, 4], [2, 5, 3, 3]),
+                                 ([2, 3, 4, 2, 3, 4], [2, 3, 4, 2, 3, 4]),
+                                 ([1, 7, 2], [6, 1, 8]), ([7, 7, 7, 6], [7, 6, 7, 7])])
+def test_vbatched_small_entries_integer(m, n):
+    """Mixed per-mode extents at d >= 3: a wave per entry while X, both
+    intermediates and the factors fit 16 KB (all but the last shape, which
+    takes the generic kernel); integer values, Y shared, exact comparison."""
+    d = len(m)
+    batch, ny = 301, 5
+    rng = np.random.default_rng(sum(m) * 100 + sum(n))
+    fs = max(a * b for a, b in zip(m, n))
+    A_pool = np.zeros((batch * d, fs))
+    for r in range(batch * d):
+        k = r % d
+        A_pool[r, :m[k] * n[k]] = rng.integers(-2, 3, m[k] * n[k])
+    sx, sy = int(np.prod(n)), int(np.prod(m))
+    X_pool = rng.integers(-3, 4, (batch, sx)).astype(np.float64)
+    y_map = (np.arange(batch) * 7) % ny
+    _, sxw = _sizes(m, n)
+    Wcap = 2 * sxw * 8 * 4
+    Y0 = np.zeros((ny, sy))
+    Yw = Y0.copy()
+    oracle.vbatched(O, m, n, A_pool, np.arange(batch * d).reshape(batch, d), X_pool,
+                    np.arange(batch), Yw, y_map, np.zeros(Wcap // 8), Wcap, batch)
+    Ad, Xd, Yd = dev(A_pool), dev(X_pool), dev(Y0)
+    Wd = torch.full((Wcap // 8,), float("nan"), dtype=torch.float64, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * d)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch)
+    assert np.array_equal(host(Yd), Yw), (m, n)
+
+
+def test_vbatched_small_entries_f32():
+    m, n = [4, 5, 6], [3, 6, 5]
+    d, batch, ny = 3, 97, 4
+    A_pool, X_pool, y_map = _vb_case(m, n, batch, 77, ny)
+    _, sxw = _sizes(m, n)
+    Wcap = 2 * sxw * 4 * 2
+    Y0 = np.zeros((ny, int(np.prod(m))))
+    Yw = Y0.copy()
+    A32, X32 = A_pool.astype(np.float32), X_pool.astype(np.float32)
+    oracle.vbatched(O, m, n, A32.astype(np.float64), np.arange(batch * d).reshape(batch, d),
+                    X32.astype(np.float64), np.arange(batch), Yw, y_map,
+                    np.zeros(Wcap // 4), Wcap * 2, batch)
+    Ad, Xd = dev(A32), dev(X32)
+    Yd = torch.zeros((ny, int(np.prod(m))), dtype=torch.float32, device=DEV)
+    Wd = torch.empty(Wcap // 4, dtype=torch.float32, device=DEV)
+    K.kronmult_vbatched(m, n, ptrs(Ad, np.arange(batch * d)), ptrs(Xd, np.arange(batch)),
+                        ptrs(Yd, y_map), Wd, Wcap, batch, dtype=torch.float32)
+    # d dot products of at most 6 terms per output, plus the atomic sums of
+    # about batch / ny entries: k = 18 + 25 roundings of 2^-24 (x8)
+    assert H.rel_err(host(Yd), Yw) < 8 * (18 + batch // ny + 1) * 2.0 ** -24
+
+
 def test_vbatched_square_headline_shape():
     m = n = [4] * 6
     batch = 100
