@@ -42,7 +42,10 @@ struct FastGeom {
     // threads instead of 4 of 64)
     static constexpr int G2 = F >= THREADS ? THREADS : pow2ceil(F);
     static constexpr bool EXACT = !BIG && F < 128 && THREADS / F > THREADS / G2;
-    static constexpr int G = EXACT ? F : G2;
+    // F in (64, 128] with no exact packing (d4n5: 125 fibers) takes a whole
+    // workgroup: one entry per workgroup reads its factors into SGPRs
+    static constexpr bool WHOLE = !BIG && !EXACT && F > 64 && F <= 128;
+    static constexpr int G = EXACT ? F : WHOLE ? THREADS : G2;
     static constexpr int P = THREADS / G;        // entries per workgroup
     static constexpr size_t lds_bytes(size_t elem) { return (size_t)P * E * elem; }
 };

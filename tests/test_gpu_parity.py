@@ -694,8 +694,8 @@ def test_kgemm_shared_c_atomic():
 
 # --- grouped (Y-merging) pointer-layout path: batch >= 2048 -------------------
 
-# shapes with several entries per workgroup ((4, 5), (4, 4), (3, 4), (3, 8),
-# (2, 8), (2, 2)) are not grouped: their per-element atomics cost less than the
+# shapes with several entries per workgroup ((4, 4), (3, 4), (3, 8), (2, 8),
+# (2, 2)) are not grouped: their per-element atomics cost less than the
 # ordering's fixed ~20 us; they run here at the same batch sizes
 @pytest.mark.parametrize("d,n", [(6, 4), (5, 4), (5, 5), (4, 5), (4, 4), (3, 4), (3, 8),
                                  (2, 8), (2, 2), (4, 6), (6, 3), (6, 5), (5, 6), (5, 7)])
