@@ -425,29 +425,32 @@ def bench_large_n(args):
 
 
 def bench_fused_shapes(args):
-    """Batched FP64 shapes of the fused kernels, one JSON line each: d = 2..6,
-    n = 2..8, every n^d from 8 to 20,480 elements; 2^25 / n^d entries (at most
-    4M), uniform [0, 1) data; algorithmic bytes X + Y read-modify-write + A."""
+    """Batched shapes of the fused kernels, one JSON line each: d = 2..6,
+    n = 2..8, every n^d from 8 to 20,480 elements (40,960 in FP32, --dtype
+    f32); 2^25 / n^d entries (at most 4M), uniform [0, 1) data; algorithmic
+    bytes X + Y read-modify-write + A."""
     import torch
     import kronmult_amd as K
     dev = "cuda:0"
     stream = torch.cuda.current_stream()
     torch.manual_seed(2024)
+    f32 = args.dtype == "f32"
+    tdt, w = (torch.float32, 4) if f32 else (torch.float64, 8)
     for d in range(2, 7):
         for n in range(2, 9):
             E = n ** d
-            if E > 20480 or E < 8:
+            if E > (40960 if f32 else 20480) or E < 8:
                 continue
             batch = max(64, min(1 << 22, (1 << 25) // E))
-            A = torch.rand(batch * d * n * n, dtype=torch.float64, device=dev)
-            X = torch.rand(batch * E, dtype=torch.float64, device=dev)
-            Y = torch.zeros(batch * E, dtype=torch.float64, device=dev)
-            W = torch.empty(batch * E, dtype=torch.float64, device=dev)
+            A = torch.rand(batch * d * n * n, dtype=tdt, device=dev)
+            X = torch.rand(batch * E, dtype=tdt, device=dev)
+            Y = torch.zeros(batch * E, dtype=tdt, device=dev)
+            W = torch.empty(batch * E, dtype=tdt, device=dev)
             ms = _time_steps(lambda: K.kronmult_batched(d, n, A, X, Y, W, batch, stream=stream),
                              args.steps, args.warmup, stream)
-            alg = 8 * (3 * E + d * n * n) * batch
+            alg = w * (3 * E + d * n * n) * batch
             print(json.dumps({"workload": "kronmult_batched fused shape", "d": d, "n": n,
-                              "dtype": "f64", "batch": batch, "ms": round(ms, 4),
+                              "dtype": args.dtype, "batch": batch, "ms": round(ms, 4),
                               "alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
                               "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}),
                   flush=True)
@@ -547,6 +550,8 @@ def main():
     # calls 0.55 -> 0.40 ms over the first ~30): warm up for ~80 ms
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="xbatched_d6n4")
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
+                    help="element type of the fused_shapes workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     args = ap.parse_args()
