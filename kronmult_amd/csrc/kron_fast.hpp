@@ -26,15 +26,18 @@
 namespace kron {
 
 // An entry whose LDS alone exceeds 80 KB leaves room for one workgroup per
-// CU; with 256 threads that is 4 waves, too few to hide the passes' LDS and
-// HBM latency.  Such entries get 1,024 threads when the factor fits the
+// CU (two above 40 KB); with 256 threads that is 4 (8) waves, too few to hide
+// the passes' LDS and HBM latency.  Such entries get 1,024 threads when the factor fits the
 // scalar registers (a thread then needs well under the 128 VGPRs of four
 // waves per SIMD), else 512 (two waves per SIMD, 256 VGPRs).
 template <typename T, int D, int N>
 struct FastGeom {
     static constexpr int E = cpow(N, D);        // elements per entry
     static constexpr int F = E / N;              // fibers per mode
-    static constexpr bool BIG = (size_t)E * sizeof(T) > 81920 && F >= 1024;
+    // FP64 from 40 KB (d5n6: 0.198 -> 0.165 ms); in FP32 the extra threads
+    // did not pay below 80 KB (d5n7, d6n5 3-5% slower)
+    static constexpr bool BIG =
+        (size_t)E * sizeof(T) > (sizeof(T) == 8 ? 40960 : 81920) && F >= 1024;
     static constexpr int THREADS = !BIG ? 256 : (size_t)N * N * sizeof(T) <= 200 ? 1024 : 512;
     // threads per entry: F rounded up to a power of two, or exactly F when
     // that packs more entries into the workgroup (F < 128 only, so P stays
