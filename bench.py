@@ -9,11 +9,18 @@ uniformly from a pool of 2,048 vectors (about 32 atomic writers per Y), fixed
 seed.  A step is one call over the whole batch, inputs resident in HBM.
 
 Printed: one JSON line with entries/s, the roofline of the dominant kernel
-(algorithmic bytes per launch / its average duration from HIP events on the
-launch stream) and the reference CPU path timed on this host's cores.
+and the reference CPU path timed on this host's cores.
 
-Algorithmic bytes per entry (xbatched): 8*(n^d [X] + 2*n^d [Y RMW] + d*n^2
-[A]) + (d+2)*8 [pointers] = 99,136 B at d=6, n=4.
+Roofline: FP64 throughput.  Each entry costs 2*d*n^(d+1) = 196,608 flops
+(n FMAs per element and mode).  The pools make the compulsory HBM footprint
+about 0.46 GB per call (X pool 268 MB, Y pool read and written 134 MB, the
+factors 50 MB, pointers 4 MB) against 12.9 GFLOP: 28 flop/B, above the FP64
+ridge of 78.6 TFLOP/s / 8 TB/s = 9.8 flop/B.  roofline.achieved is the
+launch's flops over the kernel's average duration (HIP events on its stream,
+the update run through an xbatched plan so that one launch is one kernel).
+The per-entry algorithmic bytes 8*(n^d [X] + 2*n^d [Y RMW] + d*n^2 [A]) +
+(d+2)*8 [pointers] = 99,136 B ignore the pools' reuse; they are reported
+under "hbm" with the compulsory bytes and the rocprofv3 counter bytes.
 """
 import argparse
 import json
@@ -56,20 +63,11 @@ def bytes_per_entry(d, n, ptr_layout=True, elem=8):
     return b + ((d + 2) * 8 if ptr_layout else 0)
 
 
-def cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample, min_seconds=10.0):
-    """The reference CPU path (oracle/_ref, compiled from the reference's own
-    sources) or, where absent, the oracle's C restatement, on the first
-    `sample` entries of the same workload, repeated for about `min_seconds`
-    of CPU work; the median rate is reported."""
+def _cpu_rate(L, d, n, A, X, x_map, y_map, ny, sample, subbatch, min_seconds):
+    """Median entries/s of the CPU path over repeated runs of `sample` entries."""
     import oracle
-    L = oracle.ref_lib()
-    kind = "reference"
-    if L is None:
-        L = oracle.oracle_lib()
-        kind = "port"
     E = n ** d
     Y = np.zeros((ny, E))
-    subbatch = 1024
     W = np.zeros((2 * subbatch, E))
     a_map = np.arange(sample * d).reshape(sample, d)
     times = []
@@ -79,27 +77,71 @@ def cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample, min_seconds=10.0
         oracle.xbatched(L, d, n, A, a_map, n, X, x_map[:sample], Y, y_map[:sample], W,
                         np.arange(2 * subbatch), sample, subbatch)
         times.append(time.perf_counter() - t0)
+    return sample / float(np.median(times)), times
+
+
+def cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample):
+    """The reference CPU path (oracle/_ref, compiled from the reference's own
+    sources) or, where absent, the oracle's C restatement, on the first
+    `sample` entries of the same workload (subbatchCount 1,024, as the
+    reference's xbatched loop runs it), at every host core the process may
+    use for about 10 s, then at one thread on a smaller sample for about 8 s;
+    median rates.  Threads are left unbound: OMP_PROC_BIND=close with
+    OMP_PLACES=cores put all eight threads of an 8-core container on one
+    core (6.0e3 against 3.4e4 entries/s unbound)."""
+    import oracle
+    L = oracle.ref_lib()
+    kind = "reference"
+    if L is None:
+        L = oracle.oracle_lib()
+        kind = "port"
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    med = float(np.median(times))
+    oracle.set_threads(L, threads)
+    rate, times = _cpu_rate(L, d, n, A, X, x_map, y_map, ny, sample, 1024, 10.0)
+    sample1 = max(1, sample // 16)
+    oracle.set_threads(L, 1)
+    rate1, times1 = _cpu_rate(L, d, n, A, X, x_map, y_map, ny, sample1, min(1024, sample1), 8.0)
+    oracle.set_threads(L, threads)
     return {
-        "value": sample / med,
+        "value": rate,
         "unit": "entries/s",
         "cores": threads,
         "kind": kind,
-        "sample": "the workload's first %d entries (same pools and maps), "
-                  "subbatchCount=1024, %d OpenMP threads, %d runs in %.1f s; median "
-                  "%.3f s, best %.3f s per run"
-                  % (sample, threads, len(times), sum(times), med, min(times)),
+        "sample": "the workload's first %d entries (same pools and maps), subbatchCount=1024, "
+                  "%d OpenMP threads (OMP_PROC_BIND=%s, OMP_PLACES=%s), %d runs in %.1f s; "
+                  "median %.3f s, best %.3f s per run"
+                  % (sample, threads, os.environ.get("OMP_PROC_BIND"),
+                     os.environ.get("OMP_PLACES"), len(times), sum(times),
+                     float(np.median(times)), min(times)),
+        "value_1thread": rate1,
+        "sample_1thread": "the first %d entries at 1 thread, %d runs in %.1f s; median %.3f s"
+                          % (sample1, len(times1), sum(times1), float(np.median(times1))),
     }
 
 
+def sources_sha16():
+    """Hash of the HIP library's sources: a counter profile is used only for
+    the build it measured."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "kronmult_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(workload):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of
+    this workload, or None when that summary measured other sources."""
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        rec = json.load(f)
+    if rec.get("sources_sha16") != sources_sha16():
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def _time_steps(step, steps, warmup, stream):
@@ -115,6 +157,18 @@ def _time_steps(step, steps, warmup, stream):
     ev1.record(stream)
     torch.cuda.synchronize()
     return ev0.elapsed_time(ev1) / steps
+
+
+def preheat(step, seconds=0.25):
+    """Run `step` back to back for about `seconds` before any warmup: the
+    GPU's clocks ramp up over the first tens of milliseconds under load, and
+    both timed regions should see the steady state."""
+    import torch
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(20):
+            step()
+        torch.cuda.synchronize()
 
 
 def bench_xbatched(args):
@@ -147,18 +201,28 @@ def bench_xbatched(args):
         K.kronmult_xbatched(d, n, pA, n, pX, pY, pW, batch, 0, dtype=torch.float64,
                             stream=stream)
 
+    preheat(step)
     t0 = time.perf_counter()
     ms = _time_steps(step, args.steps, args.warmup, stream)
     wall = time.perf_counter() - t0
     value = batch / (ms * 1e-3)
     # The dominant kernel alone: the same update through an xbatched plan
-    # (Y-target order computed once), timed with HIP events on its stream.
+    # (Y-target order computed once), so one launch per execute, timed with
+    # HIP events on the stream it is launched on.
     plan = K.XbatchedPlan(d, n, pA, n, pX, pY, pW, batch, dtype=torch.float64, stream=stream)
     ms_kernel = _time_steps(lambda: plan.execute(stream), args.steps, args.warmup, stream)
     plan.close()
+    # FP64 work: 2*d*n^(d+1) flops per entry (each mode: n FMAs per element),
+    # the reference's own count (test_kronmult6_batched.cpp:324-335)
+    flops = 2 * d * n ** (d + 1) * batch
+    tflops = flops / (ms_kernel * 1e-3) / 1e12
     alg_bytes = bytes_per_entry(d, n) * batch
-    achieved = alg_bytes / (ms_kernel * 1e-3) / 1e9
+    # compulsory footprint: every X and Y vector the maps name (Y read and
+    # written once), the factors and the pointer arrays
+    compulsory = 8 * E * (len(np.unique(x_map)) + 2 * len(np.unique(y_map))) + \
+        8 * d * n * n * batch + 8 * (d + 2) * batch
     traffic = read_traffic("xbatched_d6n4")
+    sec = ms_kernel * 1e-3
     out = {
         "metric": "batch entries/sec (kronmult6_xbatched d=6 n=4 FP64, batchCount 65536)",
         "value": value,
@@ -175,21 +239,31 @@ def bench_xbatched(args):
         "config": {"workload": "kronmult6_xbatched d=6 n=4 batchCount=65536 "
                                "Xpool=8192 Ypool=2048 lda=4",
                    "global_batch": batch},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        # Pooled X (8,192 vectors) and Y (2,048) make the workload's
+        # compulsory HBM footprint 0.46 GB against 12.9 GFLOP: 28 flop/B,
+        # above MI355X's FP64 ridge (78.6 TFLOP/s / 8 TB/s = 9.8), so the
+        # binding roof is FP64 throughput (the kernel's FMAs run on the
+        # vector units; 78.6 TFLOP/s is the FP64 peak of both the vector and
+        # the matrix units)
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic},
+        "kernel": "kron_plane_kernel<double,double,Ptr,true,false,true> (Y-target order)",
         "kernel_ms": ms_kernel,
-        "step_alg_gbs": alg_bytes / (ms * 1e-3) / 1e9,
-        # the kernel's measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-        # profiles/traffic_xbatched_d6n4.json) over its live duration: merging
-        # Y runs removes most of the algorithmic Y read-modify-write, so this,
-        # not roofline.frac, shows how close the kernel runs to HBM bandwidth
-        "kernel_traffic_gbs": (traffic / (ms_kernel * 1e-3) / 1e9) if traffic else None,
+        "flops_per_launch": flops,
+        "hbm": {
+            "peak_gbs": HBM_PEAK_GBS,
+            "alg_bytes": alg_bytes, "alg_gbs": alg_bytes / sec / 1e9,
+            "compulsory_bytes": compulsory, "compulsory_gbs": compulsory / sec / 1e9,
+            "compulsory_frac": compulsory / sec / 1e9 / HBM_PEAK_GBS,
+            "counter_bytes": traffic,
+            "counter_gbs": (traffic / sec / 1e9) if traffic else None,
+            "counter_frac": (traffic / sec / 1e9 / HBM_PEAK_GBS) if traffic else None,
+        },
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
-        sample = args.cpu_sample
-        cpu = cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny, sample)
-        out["cpu_baseline"] = cpu
+        out["cpu_baseline"] = cpu_baseline_xbatched(d, n, A, X, x_map, y_map, ny,
+                                                    args.cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -61,6 +61,8 @@ def oracle_lib():
         L.oracle_dense_kron_f64.restype = None
         L.oracle_fill_uniform_f64.argtypes = [_vp, ctypes.c_size_t, ctypes.c_uint64]
         L.oracle_fill_uniform_f64.restype = None
+        L.oracle_set_threads.argtypes = [_i]
+        L.oracle_set_threads.restype = None
         _cache["oracle"] = L
     return _cache["oracle"]
 
@@ -74,8 +76,19 @@ def ref_lib():
             _declare(L, "ref", False)
             L.ref_pbatched_f64.argtypes = [_i, _i, _vp, _vp, _vp, _vp, _i]
             L.ref_pbatched_f64.restype = _i
+            L.ref_set_threads.argtypes = [_i]
+            L.ref_set_threads.restype = None
         _cache["ref"] = L
     return _cache["ref"]
+
+
+def set_threads(L, threads):
+    """OpenMP thread count of an oracle library's loops (C restatement or
+    the compiled reference)."""
+    if L is oracle_lib():
+        L.oracle_set_threads(int(threads))
+    else:
+        L.ref_set_threads(int(threads))
 
 
 def _sfx(a):

@@ -34,6 +34,7 @@
 #include <stddef.h>
 #include <string.h>
 #include <stdint.h>
+#include <omp.h>
 
 #define MINI(a, b) (((a) < (b)) ? (a) : (b))
 #define MAXI(a, b) (((a) > (b)) ? (a) : (b))
@@ -342,4 +343,10 @@ void oracle_fill_uniform_f64(double *out, size_t count, uint64_t seed)
         const uint64_t k = splitmix64_next(&s) >> 11;
         out[i] = 2.0 * ((double)k * (1.0 / 9007199254740992.0)) - 1.0;
     }
+}
+
+/* Thread count of the OpenMP loops above (CPU-baseline timing). */
+void oracle_set_threads(int t)
+{
+    omp_set_num_threads(t > 0 ? t : 1);
 }

@@ -32,6 +32,8 @@
 #include "kgemm_nn_batched.hpp"
 #include "kgemm_nt_batched.hpp"
 
+#include <omp.h>
+
 // Host wrappers defined in the reference .cpp files (C++ linkage).
 #define DECL_BATCHED(D)                                                      \
     void kronmult##D##_batched(int const n, double const Aarray_[],         \
@@ -194,5 +196,9 @@ int ref_kgemm_batched_f32(char trans, int mm, int nn, int kk, float alpha,
         kgemm_nn_batched<float>(mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC, batch);
     return 0;
 }
+
+// Thread count of the reference's OpenMP loops (the CPU baseline is timed
+// at all host cores and at one thread).
+void ref_set_threads(int t) { omp_set_num_threads(t > 0 ? t : 1); }
 
 }  // extern "C"
