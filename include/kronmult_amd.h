@@ -39,7 +39,8 @@ typedef enum kron_status {
     KRON_EINVAL = 1,      /* invalid argument (d, n, sizes, null pointers) */
     KRON_EWORKSPACE = 2,  /* workspace (W / pW / Wcapacity) too small */
     KRON_EHIP = 3,        /* HIP runtime reported an error */
-    KRON_ENOTSUP = 4      /* shape outside every kernel in this build */
+    KRON_ENOTSUP = 4,     /* shape outside every kernel in this build */
+    KRON_ETOOBIG = 5      /* batchCount above KRON_NOALLOC_MAX_BATCH */
 } kron_status;
 
 typedef void *kron_stream_t; /* hipStream_t */
@@ -115,6 +116,26 @@ kron_status kron_xbatched_f32(int d, int n, const float *const *Aarray, int lda,
                               const float *const *pX, float *const *pY,
                               float *const *pW, int batchCount,
                               int subbatchCount, kron_stream_t stream);
+
+/*
+ * kron_xbatched_f64 without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under hipGraph stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and kernels as kron_xbatched_f64; the scratch of the
+ * Y-target sort (calls with batchCount >= 2048) lives in a fixed device
+ * array of the library, about 16.8 MB per GPU, held until the process ends,
+ * instead of the library's lazily allocated per-stream workspace.  The call
+ * allocates and frees nothing and never synchronises.
+ *   batchCount > KRON_NOALLOC_MAX_BATCH: KRON_ETOOBIG, nothing launched.
+ * Every call on one GPU uses that GPU's one array, so calls that overlap on
+ * one GPU (on different streams, or different threads) race; calls ordered
+ * on one stream do not.
+ */
+#define KRON_NOALLOC_MAX_BATCH 1048576
+kron_status kron_xbatched_noalloc_f64(int d, int n, const double *const *Aarray, int lda,
+                                      const double *const *pX, double *const *pY,
+                                      double *const *pW, int batchCount,
+                                      int subbatchCount, kron_stream_t stream);
 
 /*
  * Variable-shape layout: factor k is m[k] x n[k] with leading dimension m[k]

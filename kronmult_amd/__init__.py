@@ -30,7 +30,8 @@ __all__ = [
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkronmult_amd.so")
 
-KRON_OK, KRON_EINVAL, KRON_EWORKSPACE, KRON_EHIP, KRON_ENOTSUP = range(5)
+KRON_OK, KRON_EINVAL, KRON_EWORKSPACE, KRON_EHIP, KRON_ENOTSUP, KRON_ETOOBIG = range(6)
+KRON_NOALLOC_MAX_BATCH = 1048576
 
 
 class KronError(RuntimeError):
@@ -79,6 +80,8 @@ def lib():
         f.argtypes = [ctypes.POINTER(_c_vp), _c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp,
                       _c_int, _c_int, _c_vp]
         f.restype = _c_int
+    L.kron_xbatched_noalloc_f64.argtypes = _SIGS["kron_xbatched_%s"]
+    L.kron_xbatched_noalloc_f64.restype = _c_int
     L.kron_xbatched_plan_execute.argtypes = [_c_vp, _c_vp]
     L.kron_xbatched_plan_execute.restype = _c_int
     L.kron_xbatched_plan_destroy.argtypes = [_c_vp]

@@ -156,7 +156,7 @@ static kron_status pbatched_impl(int d, int n, const T *A, const T *const *pX, T
 }
 
 template <typename T>
-static kron_status xbatched_args(KronArgs<T> &a, int d, int n, const T *const *Aarray, int lda,
+kron_status xbatched_args(KronArgs<T> &a, int d, int n, const T *const *Aarray, int lda,
                                  const T *const *pX, T *const *pY, T *const *pW, int batch,
                                  int subbatch, kron_stream_t stream)
 {
@@ -185,6 +185,13 @@ static kron_status xbatched_args(KronArgs<T> &a, int d, int n, const T *const *A
     if (batch > 0 && kron_scratch_elems(a) > 0 && !pW) return KRON_EWORKSPACE;
     return KRON_OK;
 }
+
+template kron_status xbatched_args<double>(KronArgs<double> &, int, int, const double *const *,
+                                          int, const double *const *, double *const *,
+                                          double *const *, int, int, kron_stream_t);
+template kron_status xbatched_args<float>(KronArgs<float> &, int, int, const float *const *, int,
+                                         const float *const *, float *const *, float *const *,
+                                         int, int, kron_stream_t);
 
 template <typename T>
 static kron_status xbatched_impl(int d, int n, const T *const *Aarray, int lda,
@@ -363,6 +370,9 @@ const char *kron_status_string(kron_status s)
     case KRON_EWORKSPACE: return "KRON_EWORKSPACE: workspace too small or missing";
     case KRON_EHIP: return "KRON_EHIP: HIP runtime error";
     case KRON_ENOTSUP: return "KRON_ENOTSUP: shape not supported";
+    case KRON_ETOOBIG:
+        return "KRON_ETOOBIG: batchCount above the no-allocation entry point's limit "
+               "(KRON_NOALLOC_MAX_BATCH)";
     }
     return "unknown kron_status";
 }

@@ -487,7 +487,15 @@ static bool group_order(const KronArgs<T> &a, int N, GroupOrder &ord, hipStream_
     if (a.batch < kGroupMinBatch) return false;
     const size_t bytes = group_workspace_bytes(a.batch);
     bool fresh = false;
-    void *ws = stream_workspace(s, bytes, &fresh);
+    void *ws = nullptr;
+    if (a.sort_ws) {
+        // caller's scratch: its contents are whatever the last call left
+        if (a.sort_ws_bytes < bytes) return false;
+        ws = a.sort_ws;
+        fresh = true;
+    } else {
+        ws = stream_workspace(s, bytes, &fresh);
+    }
     if (!ws) return false;
     const unsigned long long *keys = nullptr;
     const int *perm = nullptr;

@@ -35,30 +35,9 @@ namespace kron {
 // the workspace is new, restored by every call's last workgroup.  The ranges
 // land in an arbitrary order; equal targets are still contiguous.
 
-// Buckets: the power of two at least the batch, within [1024, 65536].  With
-// T distinct targets about T^2 / (2 * buckets) pairs share a bucket.
-constexpr int kGroupMaxBits = 16;
-
-static int group_bits(int batch)
-{
-    int bits = 10;
-    while (bits < kGroupMaxBits && (1 << bits) < batch) ++bits;
-    return bits;
-}
-
-static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
-
-// counts (for the largest bucket count, so the zeroed region does not move
-// with the batch), then the ticket and the used-bucket count
-static constexpr size_t kGroupCountBytes = (size_t)4 << kGroupMaxBits;
-static constexpr size_t kGroupZeroBytes = kGroupCountBytes + 256;
-
-size_t group_workspace_bytes(int batch)
-{
-    const int nb = 1 << group_bits(batch);
-    return kGroupZeroBytes + 2 * align256((size_t)nb * 4) + 2 * align256((size_t)batch * 4) +
-           align256((size_t)batch * 8);
-}
+// Buckets: the power of two at least the batch, within [1024, 65536]
+// (group_bits).  With T distinct targets about T^2 / (2 * buckets) pairs
+// share a bucket.  The workspace layout is group_workspace_bytes's.
 
 // Fibonacci hashing of the pointer (targets are at least 4-byte aligned).
 __device__ __forceinline__ unsigned group_bucket(unsigned long long p, int bits)
@@ -185,13 +164,13 @@ kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t w
     int *nused = reinterpret_cast<int *>(p + kGroupCountBytes + 4);
     p += kGroupZeroBytes;
     int *base = reinterpret_cast<int *>(p);
-    p += align256((size_t)nb * 4);
+    p += group_align256((size_t)nb * 4);
     int *used = reinterpret_cast<int *>(p);
-    p += align256((size_t)nb * 4);
+    p += group_align256((size_t)nb * 4);
     int *rank = reinterpret_cast<int *>(p);
-    p += align256((size_t)batch * 4);
+    p += group_align256((size_t)batch * 4);
     int *perm_out = reinterpret_cast<int *>(p);
-    p += align256((size_t)batch * 4);
+    p += group_align256((size_t)batch * 4);
     unsigned long long *keys_out = reinterpret_cast<unsigned long long *>(p);
     const auto keys_in = reinterpret_cast<const unsigned long long *>(pY);
     if (fresh)

@@ -63,7 +63,18 @@ struct KronArgs {
     hipStream_t stream = nullptr;
     // pre-computed Y-target order (an xbatched plan); nullptr -> decide per call
     const GroupOrder *order = nullptr;
+    // caller-provided device scratch for the Y-target sort (cleared by every
+    // call that uses it); nullptr -> the library's per-stream workspace
+    void *sort_ws = nullptr;
+    size_t sort_ws_bytes = 0;
 };
+
+// Validates kron_xbatched_* arguments and fills `a` (the reference's
+// subbatchCount and pW slot rules); KRON_OK or the status to return.
+template <typename T>
+kron_status xbatched_args(KronArgs<T> &a, int d, int n, const T *const *Aarray, int lda,
+                          const T *const *pX, T *const *pY, T *const *pW, int batch,
+                          int subbatch, kron_stream_t stream);
 
 // d = 1 entries (a batched matrix-vector product, kron_gemv.hip): KRON_ENOTSUP
 // for other d or transposed factors.
@@ -105,7 +116,25 @@ kron_status launch_kgemm(bool transB, int mm, int nn, int kk, T alpha,
 // Y-target ordering of pointer-layout entries (kron_group.hip).  The start
 // of the workspace must be zero before the first call on it (pass fresh =
 // true for a new workspace: the call clears it); every call leaves it zero.
-size_t group_workspace_bytes(int batch);
+// Workspace: the bucket counts (sized for the largest bucket count, so the
+// zeroed region does not move with the batch), the finishing ticket and the
+// used-bucket count, then bucket bases and the used-bucket list, per-entry
+// ranks, the order and the sorted keys.
+constexpr int kGroupMaxBits = 16;
+constexpr int group_bits(int batch)
+{
+    int bits = 10;
+    while (bits < kGroupMaxBits && (1 << bits) < batch) ++bits;
+    return bits;
+}
+constexpr size_t group_align256(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr size_t kGroupCountBytes = (size_t)4 << kGroupMaxBits;
+constexpr size_t kGroupZeroBytes = kGroupCountBytes + 256;
+constexpr size_t group_workspace_bytes(int batch)
+{
+    return kGroupZeroBytes + 2 * group_align256((size_t)(1 << group_bits(batch)) * 4) +
+           2 * group_align256((size_t)batch * 4) + group_align256((size_t)batch * 8);
+}
 kron_status group_by_target(const void *const *pY, int batch, void *ws, size_t ws_bytes,
                             bool fresh, const unsigned long long **keys_sorted, const int **perm,
                             hipStream_t stream);
