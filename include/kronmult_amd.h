@@ -15,8 +15,17 @@
  *     memory owned by the caller; calls never synchronise the stream;
  *   - pointer-layout calls with batchCount >= 2048 order their entries by Y
  *     target on the device and merge runs before the atomic adds; the sort
- *     scratch (about 12 bytes per entry) is a library-owned device buffer per
+ *     scratch (about 16 bytes per entry) is a library-owned device buffer per
  *     (device, stream), allocated on first use and kept for the process;
+ *   - which calls allocate device memory (hipMalloc, never hipFree, never a
+ *     synchronisation): kron_xbatched_* / kron_pbatched_* / kron_vbatched_*
+ *     when the stream's sort workspace must grow (the first grouped call on
+ *     a stream, or a larger batch than before), kron_pbatched_* when its
+ *     factor-pointer table must grow, and the plan create calls.  A call
+ *     that allocates must not be made under stream capture; warm the stream
+ *     up with an uncaptured call of the largest batch first, or use
+ *     kron_xbatched_noalloc_f64, or a plan.  Outgrown workspaces are kept
+ *     until the process ends, so graphs captured earlier stay valid;
  *   - X is never written and W is used only as scratch when a shape does not
  *     fit on chip (the reference clobbers X in its batched layouts; callers
  *     must still treat X/W contents as unspecified after a call there).

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <tuple>
 #include <utility>
+#include <vector>
 
 #include "kron_internal.hpp"
 #include "kron_mixed.hpp"
@@ -66,24 +67,22 @@ void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot, bool *fr
     };
     static std::mutex mu;
     static std::map<std::tuple<int, hipStream_t, int>, Buf> bufs;
+    // Outgrown buffers are never freed: queued work on the stream, or a graph
+    // captured earlier, may still use them.  Sizes double, so the retired
+    // buffers of a slot add up to less than its live one.
+    static std::vector<void *> retired;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     std::lock_guard<std::mutex> lk(mu);
     Buf &b = bufs[std::make_tuple(dev, stream, slot)];
     if (fresh) *fresh = false;
     if (b.bytes < bytes) {
-        // the old buffer may still be in use by queued work on this stream
-        if (b.ptr) {
-            if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
-            (void)hipFree(b.ptr);
-            b.ptr = nullptr;
-            b.bytes = 0;
-        }
-        size_t want = bytes + bytes / 4;
-        if (hipMalloc(&b.ptr, want) != hipSuccess) {
-            b.ptr = nullptr;
-            return nullptr;
-        }
+        size_t want = b.bytes ? 2 * b.bytes : bytes + bytes / 4;
+        while (want < bytes) want *= 2;
+        void *p = nullptr;
+        if (hipMalloc(&p, want) != hipSuccess) return nullptr;
+        if (b.ptr) retired.push_back(b.ptr);
+        b.ptr = p;
         b.bytes = want;
         if (fresh) *fresh = true;
     }
