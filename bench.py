@@ -201,7 +201,7 @@ def bench_xbatched(args):
         K.kronmult_xbatched(d, n, pA, n, pX, pY, pW, batch, 0, dtype=torch.float64,
                             stream=stream)
 
-    preheat(step)
+    preheat(step, args.preheat)
     t0 = time.perf_counter()
     ms = _time_steps(step, args.steps, args.warmup, stream)
     wall = time.perf_counter() - t0
@@ -627,6 +627,8 @@ def main():
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"],
                     help="element type of the fused_shapes workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--preheat", type=float, default=0.25,
+                    help="seconds of back-to-back headline calls before the warmup")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:

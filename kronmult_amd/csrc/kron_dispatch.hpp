@@ -33,4 +33,11 @@ kron_status launch_mfma_ptr(int d, int n, const Entries<T, Layout::Ptr> &e,
                             const ScratchDesc<T> *scr, int group, int lda, int batch,
                             hipStream_t s);
 
+// Square entries larger than LDS with n <= 6 (kron_split.hip): d = 6, n = 6.
+// Two passes through the caller's scratch (scr), `group` entries at a time
+// (entries of one group have distinct scratch vectors).
+bool split_supported(int d, int n);
+template <typename T>
+kron_status launch_split(const KronArgs<T> &a, const ScratchDesc<T> &scr, int group);
+
 }  // namespace kron

@@ -605,6 +605,18 @@ kron_status launch_kron(const KronArgs<T> &a)
         kron_status st = fast_dispatch(a, a.n[0], a.stream);
         if (st != KRON_ENOTSUP) return st;
     }
+    // square entries larger than LDS with factors in SGPRs (d = 6, n = 6):
+    // two compile-time passes through the scratch
+    if (square && !a.trans && split_supported(a.d, a.n[0])) {
+        long long E = 1;
+        for (int k = 0; k < a.d; ++k) E *= a.n[0];
+        ScratchDesc<T> scr{Scratch::None, nullptr, 0, nullptr, nullptr, 0};
+        int group = a.batch;
+        kron_status st = scratch_of(a, E, scr, group);
+        if (st != KRON_OK) return st;
+        st = launch_split<T>(a, scr, group);
+        if (st != KRON_ENOTSUP) return st;
+    }
     // n >= 16: every mode a dense GEMM on the matrix cores
     if (square && !a.trans && a.layout == Layout::Batched && a.lda <= a.n[0] &&
         mfma_supported(a.d, a.n[0], nullptr)) {

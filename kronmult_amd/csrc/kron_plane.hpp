@@ -283,12 +283,22 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
         load_factor<T, Acc, 4, TRANS, true>(a, factor(cur, 2), lda);
         plane_first(v, a);
         load_factor<T, Acc, 4, TRANS, true>(a, factor(cur, 0), lda);
-        plane_second(v, a);
         if constexpr (GROUPED) {
+            // the run's running sum is the mode-1 contraction's starting
+            // value (no separate adds)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
+                Acc t[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[i][q] += v[i][q];
+                for (int q = 0; q < 4; ++q) {
+                    Acc s = __builtin_fma(a[q][0], v[i][0], acc[i][q]);
+#pragma unroll
+                    for (int k = 1; k < 4; ++k) s = __builtin_fma(a[q][k], v[i][k], s);
+                    t[q] = s;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[i][q] = t[q];
+            }
             if (!more || win.at(ent, j + 1, wb, rbeg, rstep).y != cur.y) {
                 const auto y = to_global(reinterpret_cast<T *>(cur.y)) + eC;
 #pragma unroll
@@ -300,6 +310,7 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
                     }
             }
         } else {
+            plane_second(v, a);
             const auto y = to_global(reinterpret_cast<T *>(cur.y)) + eC;
             if constexpr (SHARED) {
 #pragma unroll
