@@ -704,6 +704,15 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
     hipLaunchKernelGGL(kron_fill_zero<T>, dim3((unsigned)std::min<long long>((nA + 255) / 256, 4096)),
                        dim3(256), 0, stream, dA, nA);
     const long long fixed = adj_fixed_bytes(d, n, (int)sizeof(T));
+    // d = 6, n = 6: compile-time kernels for the three passes
+    if (d == 6 && n == 6 && (long long)batch * 36 <= 0x7fffffffLL) {
+        for (int phase = 1; phase <= 3; ++phase) {
+            const kron_status st = launch_adjoint_rows6<T>(phase, A, X, dY, dX, dA, W, batch,
+                                                           stream);
+            if (st != KRON_OK) return st;
+        }
+        return KRON_OK;
+    }
     for (int phase = 1; phase <= 3; ++phase) {
         if (phase != 2 && nf == 0) continue;
         const long long chunk = phase == 1   ? (long long)sh.C1 * sh.EF

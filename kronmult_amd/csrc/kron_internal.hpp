@@ -160,6 +160,13 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
                            int batch, hipStream_t stream);
 template <typename T>
 size_t adjoint_scratch_elems(int d, int n);
+// The d = 6, n = 6 adjoint (kron_adjoint_rows.hip), pass 1 (V into W's
+// first vector of each entry), 2 (U, dX, dA_1, dA_2) or 3 (dA_3..dA_6); the
+// dA sums are added into dA (cleared beforehand).  KRON_ENOTSUP when
+// batchCount * 36 exceeds the int range.
+template <typename T>
+kron_status launch_adjoint_rows6(int pass, const T *A, const T *X, const T *dY, T *dX, T *dA,
+                                 T *W, int batch, hipStream_t stream);
 
 // The reference's per-entry vbatched structure (kron_plain.hip): one
 // workgroup per entry, every mode a pass through the entry's scratch slot of
