@@ -91,6 +91,15 @@ kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *
                                      int batchCount, kron_stream_t stream);
 
 /*
+ * Scratch the adjoint needs (no counterpart in the reference): elements of W
+ * per entry for kron_batched_adjoint_f64 (elem_bytes 8) or _f32 (4) at
+ * (d, n) -- 0 when the entry fits on chip, else 2*n^d (two vectors per entry
+ * stream through W).  KRON_ENOTSUP for shapes without an adjoint kernel.
+ */
+kron_status kron_batched_adjoint_workspace(int d, int n, int elem_bytes,
+                                           size_t *elems_per_entry);
+
+/*
  * Contiguous factors, pointer vectors.  Replaces kronmult{1..6}_pbatched
  * (kronmult6_pbatched.cpp:5-26, kernel kronmult6_pbatched.hpp:14-81).
  *   A: (n, n, d, batchCount) as in kron_batched_*

@@ -21,7 +21,8 @@ import os
 
 __all__ = [
     "KronError", "lib", "ptr_array", "XbatchedPlan", "MixedPlan", "MixedClass",
-    "kronmult_batched", "kronmult_batched_adjoint", "kronmult_xbatched", "kronmult_vbatched",
+    "kronmult_batched", "kronmult_batched_adjoint", "adjoint_workspace_elems",
+    "kronmult_xbatched", "kronmult_vbatched",
     "kronmult_vbatched_plain", "kronmult_pbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
 ] + ["kronmult%d_%s" % (d, k) for d in range(1, 7)
@@ -98,6 +99,9 @@ def lib():
         f = getattr(L, "kron_mixed_plan_" + name)
         f.argtypes = args
         f.restype = _c_int
+    L.kron_batched_adjoint_workspace.argtypes = [_c_int, _c_int, _c_int,
+                                                 ctypes.POINTER(ctypes.c_size_t)]
+    L.kron_batched_adjoint_workspace.restype = _c_int
     L.kron_status_string.argtypes = [_c_int]
     L.kron_status_string.restype = ctypes.c_char_p
     L.kron_last_hip_error.restype = _c_int
@@ -168,6 +172,16 @@ def kronmult_batched_adjoint(d, n, Aarray, Xarray, dYarray, dXarray, dAarray, Wa
     _check("kronmult%d_batched_adjoint" % d,
            f(d, n, _addr(Aarray), _addr(Xarray), _addr(dYarray), _addr(dXarray),
              _addr(dAarray), _addr(Warray), batchCount, _stream(stream)))
+
+
+def adjoint_workspace_elems(d, n, dtype):
+    """Elements of W per entry that kronmult_batched_adjoint needs at (d, n):
+    0 when the entry fits on chip (W may be None), else 2 * n^d."""
+    out = ctypes.c_size_t()
+    _check("kron_batched_adjoint_workspace",
+           lib().kron_batched_adjoint_workspace(d, n, 8 if _sfx(dtype) == "f64" else 4,
+                                                ctypes.byref(out)))
+    return out.value
 
 
 def kronmult_pbatched(d, n, Aarray, Xarray, Yarray, Warray, batchCount, dtype=None,

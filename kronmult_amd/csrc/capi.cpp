@@ -395,6 +395,18 @@ kron_status kron_batched_adjoint_f64(int d, int n, const double *A, const double
 {
     return adjoint_impl<double>(d, n, A, X, dY, dX, dA, W, batch, s);
 }
+kron_status kron_batched_adjoint_workspace(int d, int n, int elem_bytes, size_t *elems)
+{
+    long long E = 0;
+    if (!elems || d < 1 || d > kMaxDim || n < 1 || !ipow_ok(n, d, E) ||
+        (elem_bytes != 8 && elem_bytes != 4))
+        return KRON_EINVAL;
+    const size_t need = elem_bytes == 8 ? adjoint_scratch_elems<double>(d, n)
+                                        : adjoint_scratch_elems<float>(d, n);
+    if (need == (size_t)-1) return KRON_ENOTSUP;
+    *elems = need;
+    return KRON_OK;
+}
 kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *X,
                                      const float *dY, float *dX, float *dA, float *W, int batch,
                                      kron_stream_t s)

@@ -6,12 +6,14 @@ A: (batch, d, n, n) with A[b, k] the k-th factor of entry b as a matrix
    (row i, column j), X: (batch, n^d) with the reference's index convention
    (A1 acts on the slowest index).  Y: (batch, n^d), Y[b] = kron(A[b,0], ...,
    A[b,d-1]) @ X[b].  Forward runs kron_batched_*, backward the hand-written
-   adjoint kron_batched_adjoint_* (dX and every dA_k, intermediates
-   recomputed, never stored).  FP64 and FP32, on the GPU only.
+   adjoint kron_batched_adjoint_* (dX and every dA_k).  Entries that fit on
+   chip keep every intermediate there; larger ones (d = 6, n = 6 among them)
+   stream two vectors per entry through a scratch W, allocated only then.
+   FP64 and FP32, on the GPU only.
 """
 import torch
 
-from . import kronmult_batched, kronmult_batched_adjoint
+from . import adjoint_workspace_elems, kronmult_batched, kronmult_batched_adjoint
 
 
 def _colmajor(A):
@@ -45,7 +47,8 @@ class KronMatvec(torch.autograd.Function):
         dYc = dY.contiguous()
         dX = torch.empty_like(Xc)
         dA = torch.empty_like(Ac)
-        W = torch.empty(2 * Xc.numel(), dtype=Xc.dtype, device=Xc.device)
+        per = adjoint_workspace_elems(d, n, Xc.dtype)
+        W = torch.empty(per * batch, dtype=Xc.dtype, device=Xc.device) if per else None
         kronmult_batched_adjoint(d, n, Ac, Xc, dYc, dX, dA, W, batch)
         return dA.transpose(-1, -2), dX
 

@@ -102,3 +102,15 @@ def test_host_side_validation(lib):
                                          fake, 0, None) == 0
     assert lib.kron_status_string(2).decode().startswith("KRON_EWORKSPACE")
     assert lib.kron_version() >= 100
+
+
+def test_adjoint_workspace_query(lib):
+    """Scratch of the adjoint per entry: none when the entry fits on chip,
+    two vectors when it streams (d = 6, n = 6); argument errors on the host."""
+    n_out = ctypes.c_size_t()
+    assert lib.kron_batched_adjoint_workspace(6, 6, 8, ctypes.byref(n_out)) == 0
+    assert n_out.value == 2 * 6 ** 6
+    assert lib.kron_batched_adjoint_workspace(2, 3, 8, ctypes.byref(n_out)) == 0
+    assert n_out.value == 0
+    assert lib.kron_batched_adjoint_workspace(7, 2, 8, ctypes.byref(n_out)) == 1
+    assert lib.kron_batched_adjoint_workspace(3, 2, 2, ctypes.byref(n_out)) == 1
