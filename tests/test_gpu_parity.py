@@ -676,6 +676,46 @@ def test_kgemm_small_outputs_packed(mm, nn, kk, batch, trans, beta):
     assert H.rel_err(got, Cw) < 1e-12, (mm, nn, kk)
 
 
+def _ref_kgemm_operand(nrow, ncol, ib, recip):
+    """test_kgemm_nn_batched.cpp:195-213: entry ib's operand, 1-based (i, j),
+    column-major with ld = nrow: 1 + i + j*nrow + ib, or its reciprocal."""
+    i = np.arange(1, nrow + 1)[:, None]
+    j = np.arange(1, ncol + 1)[None, :]
+    v = 1.0 + i + j * nrow + ib
+    return (1.0 / v if recip else v).T.reshape(-1)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_kgemm_reference_sweep(trans):
+    """The reference's whole sweep (test_kgemm_nn_batched.cpp:534-548 and
+    test_kgemm_nt_batched.cpp:534-548): batchCount {1, 8, 15} x kk, nn, mm in
+    1..65 step 7, its closed-form operands, C = 1, alpha = 1.3, beta = 1.2.
+    It passes max_abserr < 1e-9 against its host product; here every entry
+    of C is within 1e-12 (relative) of the oracle."""
+    alpha, beta = 1.3, 1.2
+    sizes = range(1, 66, 7)
+    for batch in (1, 8, 15):
+        for kk in sizes:
+            for nn in sizes:
+                for mm in sizes:
+                    brow, bcol = (kk, nn) if trans == "N" else (nn, kk)
+                    A = np.stack([_ref_kgemm_operand(mm, kk, b, False) for b in range(batch)])
+                    B = np.stack([_ref_kgemm_operand(brow, bcol, b, True) for b in range(batch)])
+                    C0 = np.ones((batch, mm * nn))
+                    Cw = C0.copy()
+                    oracle.kgemm(O, trans, mm, nn, kk, alpha, A, [mm] * batch, B,
+                                 [brow] * batch, beta, Cw, [mm] * batch, batch)
+                    Ad, Bd, Cd = dev(A), dev(B), dev(C0)
+                    f = K.kgemm_nn_batched if trans == "N" else K.kgemm_nt_batched
+                    f(mm, nn, kk, alpha, ptrs(Ad, np.arange(batch)), ints([mm] * batch),
+                      ptrs(Bd, np.arange(batch)), ints([brow] * batch), beta,
+                      ptrs(Cd, np.arange(batch)), ints([mm] * batch), batch)
+                    got = host(Cd)
+                    assert np.max(np.abs(got - Cw)) < 1e-9 * max(1.0, np.max(np.abs(Cw))), \
+                        (batch, mm, nn, kk)
+                    assert H.rel_err(got, Cw) < 1e-12, (batch, mm, nn, kk)
+
+
 def test_kgemm_shared_c_atomic():
     """beta == 1 with every entry naming the same C: atomic accumulation."""
     mm, nn, kk, batch = 16, 16, 4, 200
