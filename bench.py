@@ -313,6 +313,21 @@ def bench_adjoint(args):
     ms_a = _time_steps(lambda: K.kronmult_batched_adjoint(d, n, A, X, dY, dX, dA, W, batch,
                                                           stream=stream),
                        args.steps, args.warmup, stream)
+    # the pointer layout with private entries (identity maps): dX and dA are
+    # added into zeroed buffers
+    def rows(t, idx):
+        base = t.data_ptr()
+        rb = t.stride(0) * t.element_size()
+        return torch.from_numpy(base + np.asarray(idx, dtype=np.int64) * rb).to(dev)
+
+    A2, X2, dY2 = A.view(batch * d, -1), X.view(batch, E), dY.view(batch, E)
+    dX2, dA2 = torch.zeros_like(X2), torch.zeros_like(A2)
+    ids, fids = np.arange(batch), np.arange(batch * d)
+    pA, pX, pdY = rows(A2, fids), rows(X2, ids), rows(dY2, ids)
+    pdX, pdA = rows(dX2, ids), rows(dA2, fids)
+    ms_x = _time_steps(lambda: K.kronmult_xbatched_adjoint(d, n, pA, n, pX, pdY, pdX, pdA, n, W,
+                                                           batch, stream=stream),
+                       args.steps, args.warmup, stream)
     # the same through the torch.autograd wrapper: Y = kron_matvec(A, X) (a
     # fresh zeroed Y, as the kernel accumulates), then Y.backward(dY)
     from kronmult_amd.torch_ops import kron_matvec
@@ -333,10 +348,64 @@ def bench_adjoint(args):
     adj_bytes = 8 * (3 * E + 2 * d * n * n) * batch  # X, dY read, dX written, A read, dA written
     print(json.dumps({"workload": "forward + adjoint d=6 n=6 batchCount=4096", "dtype": "f64",
                       "forward_ms": ms_f, "adjoint_ms": ms_a, "adjoint_over_forward": ms_a / ms_f,
+                      "xbatched_adjoint_ms": ms_x,
                       "forward_alg_gbs": fwd_bytes / (ms_f * 1e-3) / 1e9,
                       "adjoint_alg_gbs": adj_bytes / (ms_a * 1e-3) / 1e9,
                       "autograd_forward_ms": ms_wf, "autograd_forward_backward_ms": ms_wfb,
                       "autograd_backward_ms": ms_wfb - ms_wf}), flush=True)
+
+
+def bench_xadjoint(args):
+    """Forward and pointer-layout adjoint of the pooled d=6 n=4 workload:
+    65,536 entries, X pool 8,192, Y pool 2,048, FP64.  The adjoint adds
+    K^T dY into the X-pool gradient and each entry's factor gradients."""
+    import torch
+
+    import kronmult_amd as K
+
+    d, n, batch, nx, ny = 6, 4, 65536, 8192, 2048
+    E = n ** d
+    dev = "cuda:0"
+    A, X, x_map, y_map = make_xbatched_inputs(d, n, batch, nx, ny, seed=2024)
+    Ad = torch.from_numpy(A).to(dev)
+    Xd = torch.from_numpy(X).to(dev)
+    Yd = torch.zeros((ny, E), dtype=torch.float64, device=dev)
+    dYd = torch.from_numpy(_splitmix_uniform(ny * E, 77).reshape(ny, E)).to(dev)
+    dXd = torch.zeros_like(Xd)
+    dAd = torch.zeros_like(Ad)
+    per = K.adjoint_workspace_elems(d, n, torch.float64)
+    W = torch.empty(max(per, 1) * batch, dtype=torch.float64, device=dev)
+
+    def rows(t, idx):
+        base = t.data_ptr()
+        rb = t.stride(0) * t.element_size()
+        return torch.from_numpy(base + np.asarray(idx, dtype=np.int64) * rb).to(dev)
+
+    pA, pdA = rows(Ad, np.arange(batch * d)), rows(dAd, np.arange(batch * d))
+    pX, pdX = rows(Xd, x_map), rows(dXd, x_map)
+    pY, pdY = rows(Yd, y_map), rows(dYd, y_map)
+    Wd = torch.empty((1, E), dtype=torch.float64, device=dev)
+    pW = rows(Wd, np.zeros(batch, dtype=np.int64))
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        K.kronmult_xbatched(d, n, pA, n, pX, pY, pW, batch, 0, dtype=torch.float64,
+                            stream=stream)
+
+    def adj():
+        K.kronmult_xbatched_adjoint(d, n, pA, n, pX, pdY, pdX, pdA, n, W if per else None,
+                                    batch, dtype=torch.float64, stream=stream)
+
+    preheat(fwd, args.preheat)
+    ms_f = _time_steps(fwd, args.steps, args.warmup, stream)
+    ms_a = _time_steps(adj, args.steps, args.warmup, stream)
+    flops_f = 2 * d * n ** (d + 1) * batch
+    print(json.dumps({"workload": "xbatched forward + adjoint d=6 n=4 batchCount=65536 "
+                                  "Xpool=8192 Ypool=2048", "dtype": "f64",
+                      "forward_ms": ms_f, "adjoint_ms": ms_a,
+                      "adjoint_over_forward": ms_a / ms_f,
+                      "forward_tflops": flops_f / (ms_f * 1e-3) / 1e12,
+                      "adjoint_scratch_elems_per_entry": per}), flush=True)
 
 
 def bench_mixed(args):
@@ -641,6 +710,8 @@ def main():
         bench_batched(args)
     elif args.workload == "adjoint_d6n6":
         bench_adjoint(args)
+    elif args.workload == "xadjoint_d6n4":
+        bench_xadjoint(args)
     elif args.workload == "large_n":
         bench_large_n(args)
     elif args.workload == "mixed_sparse_grid":

@@ -92,9 +92,10 @@ kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *
 
 /*
  * Scratch the adjoint needs (no counterpart in the reference): elements of W
- * per entry for kron_batched_adjoint_f64 (elem_bytes 8) or _f32 (4) at
- * (d, n) -- 0 when the entry fits on chip, else 2*n^d (two vectors per entry
- * stream through W).  KRON_ENOTSUP for shapes without an adjoint kernel.
+ * per entry for kron_batched_adjoint_f64 (elem_bytes 8) or _f32 (4) and
+ * kron_xbatched_adjoint_* at (d, n) -- 0 when the entry fits on chip, else
+ * 2*n^d (two vectors per entry stream through W).  KRON_ENOTSUP for shapes
+ * without an adjoint kernel.
  */
 kron_status kron_batched_adjoint_workspace(int d, int n, int elem_bytes,
                                            size_t *elems_per_entry);
@@ -134,6 +135,29 @@ kron_status kron_xbatched_f32(int d, int n, const float *const *Aarray, int lda,
                               const float *const *pX, float *const *pY,
                               float *const *pW, int batchCount,
                               int subbatchCount, kron_stream_t stream);
+
+/*
+ * Adjoint of the pointer-array layout (no counterpart in the reference; the
+ * layout of kronmult_xbatched.hpp:79-83).  With K_b = kron(A_1, ..., A_d) of
+ * entry b (factors Aarray[k + b*d], leading dimension lda) and the upstream
+ * gradient of its Y target at pdY[b] (entries sharing a Y target share its
+ * gradient), for every entry:
+ *   pdX[b]          += K_b^T pdY[b]                 (vector of n^d)
+ *   pdA[k + b*d]    += d<pdY[b], K_b pX[b]>/dA_k    (n x n, leading dim ldda)
+ * Results are added (device atomics), never overwritten: entries may share
+ * X vectors, their gradients and factors, and the caller clears the targets
+ * where a sum should start from zero.  W: kron_batched_adjoint_workspace's
+ * elements per entry times batchCount, contiguous (entry b at b times that);
+ * NULL when that is 0.
+ */
+kron_status kron_xbatched_adjoint_f64(int d, int n, const double *const *Aarray, int lda,
+                                      const double *const *pX, const double *const *pdY,
+                                      double *const *pdX, double *const *pdA, int ldda,
+                                      double *W, int batchCount, kron_stream_t stream);
+kron_status kron_xbatched_adjoint_f32(int d, int n, const float *const *Aarray, int lda,
+                                      const float *const *pX, const float *const *pdY,
+                                      float *const *pdX, float *const *pdA, int ldda,
+                                      float *W, int batchCount, kron_stream_t stream);
 
 /*
  * kron_xbatched_f64 without any allocation.  No counterpart in the reference:

@@ -22,6 +22,7 @@ import os
 __all__ = [
     "KronError", "lib", "ptr_array", "XbatchedPlan", "MixedPlan", "MixedClass",
     "kronmult_batched", "kronmult_batched_adjoint", "adjoint_workspace_elems",
+    "kronmult_xbatched_adjoint",
     "kronmult_xbatched", "kronmult_vbatched",
     "kronmult_vbatched_plain", "kronmult_pbatched",
     "kgemm_nn_batched", "kgemm_nt_batched",
@@ -55,6 +56,8 @@ _SIGS = {
                                _c_vp],
     "kron_batched_adjoint_%s": [_c_int, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int,
                                 _c_vp],
+    "kron_xbatched_adjoint_%s": [_c_int, _c_int, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_int,
+                                 _c_vp, _c_int, _c_vp],
 }
 
 
@@ -174,9 +177,26 @@ def kronmult_batched_adjoint(d, n, Aarray, Xarray, dYarray, dXarray, dAarray, Wa
              _addr(dAarray), _addr(Warray), batchCount, _stream(stream)))
 
 
+def kronmult_xbatched_adjoint(d, n, Aarray, lda, Xarray, dYarray, dXarray, dAarray, ldda,
+                              Warray, batchCount, dtype=None, stream=None):
+    """Adjoint of kronmult{d}_xbatched (no counterpart in the reference), over
+    int64 device pointer arrays in the layout of kronmult_xbatched.hpp:79-83:
+    dXarray[b] += kron(A)^T dYarray[b] and dAarray[k + b*d] (n x n, leading
+    dimension ldda) += the gradient of factor k, with device atomics (entries
+    may share any target).  Warray: adjoint_workspace_elems(d, n) *
+    batchCount contiguous elements, or None when that is 0."""
+    import torch
+    sfx = _sfx(dtype if dtype is not None else torch.float64)
+    f = getattr(lib(), "kron_xbatched_adjoint_" + sfx)
+    _check("kronmult%d_xbatched_adjoint" % d,
+           f(d, n, _addr(Aarray), lda, _addr(Xarray), _addr(dYarray), _addr(dXarray),
+             _addr(dAarray), ldda, _addr(Warray), batchCount, _stream(stream)))
+
+
 def adjoint_workspace_elems(d, n, dtype):
-    """Elements of W per entry that kronmult_batched_adjoint needs at (d, n):
-    0 when the entry fits on chip (W may be None), else 2 * n^d."""
+    """Elements of W per entry that kronmult_batched_adjoint and
+    kronmult_xbatched_adjoint need at (d, n): 0 when the entry fits on chip
+    (W may be None), else 2 * n^d."""
     out = ctypes.c_size_t()
     _check("kron_batched_adjoint_workspace",
            lib().kron_batched_adjoint_workspace(d, n, 8 if _sfx(dtype) == "f64" else 4,

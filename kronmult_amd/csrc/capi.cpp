@@ -273,7 +273,49 @@ static kron_status adjoint_impl(int d, int n, const T *A, const T *X, const T *d
     const size_t need = adjoint_scratch_elems<T>(d, n);
     if (need == (size_t)-1) return KRON_ENOTSUP;
     if (need > 0 && !W) return KRON_EWORKSPACE;
-    return launch_adjoint<T>(d, n, A, X, dY, dX, dA, W, batch, static_cast<hipStream_t>(stream));
+    AdjointArgs<T> a;
+    a.layout = Layout::Batched;
+    a.d = d;
+    a.n = n;
+    a.batch = batch;
+    a.A = A;
+    a.X = X;
+    a.dY = dY;
+    a.dX = dX;
+    a.dA = dA;
+    a.W = W;
+    a.stream = static_cast<hipStream_t>(stream);
+    return launch_adjoint<T>(a);
+}
+
+template <typename T>
+static kron_status xadjoint_impl(int d, int n, const T *const *pA, int lda, const T *const *pX,
+                                 const T *const *pdY, T *const *pdX, T *const *pdA, int ldda,
+                                 T *W, int batch, kron_stream_t stream)
+{
+    long long E = 0;
+    if (d < 1 || d > kMaxDim || n < 1 || lda < n || ldda < n || batch < 0 || !ipow_ok(n, d, E))
+        return KRON_EINVAL;
+    if (batch == 0) return KRON_OK;
+    if (!pA || !pX || !pdY || !pdX || !pdA) return KRON_EINVAL;
+    const size_t need = adjoint_scratch_elems<T>(d, n);
+    if (need == (size_t)-1) return KRON_ENOTSUP;
+    if (need > 0 && !W) return KRON_EWORKSPACE;
+    AdjointArgs<T> a;
+    a.layout = Layout::Ptr;
+    a.d = d;
+    a.n = n;
+    a.batch = batch;
+    a.pA = pA;
+    a.lda = lda;
+    a.pX = pX;
+    a.pdY = pdY;
+    a.pdX = pdX;
+    a.pdA = pdA;
+    a.ldda = ldda;
+    a.W = W;
+    a.stream = static_cast<hipStream_t>(stream);
+    return launch_adjoint<T>(a);
 }
 
 template <typename T>
@@ -412,6 +454,20 @@ kron_status kron_batched_adjoint_f32(int d, int n, const float *A, const float *
                                      kron_stream_t s)
 {
     return adjoint_impl<float>(d, n, A, X, dY, dX, dA, W, batch, s);
+}
+kron_status kron_xbatched_adjoint_f64(int d, int n, const double *const *Aarray, int lda,
+                                      const double *const *pX, const double *const *pdY,
+                                      double *const *pdX, double *const *pdA, int ldda,
+                                      double *W, int batch, kron_stream_t s)
+{
+    return xadjoint_impl<double>(d, n, Aarray, lda, pX, pdY, pdX, pdA, ldda, W, batch, s);
+}
+kron_status kron_xbatched_adjoint_f32(int d, int n, const float *const *Aarray, int lda,
+                                      const float *const *pX, const float *const *pdY,
+                                      float *const *pdX, float *const *pdA, int ldda, float *W,
+                                      int batch, kron_stream_t s)
+{
+    return xadjoint_impl<float>(d, n, Aarray, lda, pX, pdY, pdX, pdA, ldda, W, batch, s);
 }
 kron_status kron_xbatched_f64(int d, int n, const double *const *A, int lda,
                               const double *const *pX, double *const *pY, double *const *pW,

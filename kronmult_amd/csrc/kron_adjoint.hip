@@ -28,7 +28,7 @@
 #include <algorithm>
 #include <cmath>
 
-#include "kron_device.hpp"
+#include "kron_adjoint.hpp"
 
 namespace kron {
 
@@ -118,7 +118,7 @@ __device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *_
 // the fibers t, t + 256, ...
 template <typename T, int N, bool TRANS, typename FacP>
 __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__restrict__ dst,
-                                             int outer, int inner, FacP Al)
+                                             int outer, int inner, FacP Al, int lda)
 {
     const int fibers = outer * inner;
     if ((int)threadIdx.x >= fibers) return;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__res
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) a[i][j] = TRANS ? Al[j + i * N] : Al[i + j * N];
+        for (int j = 0; j < N; ++j) a[i][j] = TRANS ? Al[j + i * lda] : Al[i + j * lda];
     const FastDiv di = fast_div(inner);
     for (int f = threadIdx.x; f < fibers; f += 256) {
         int ii;
@@ -180,11 +180,9 @@ __device__ __forceinline__ int slab_map(const SlabMap &m, int q, int &w)
 // data also waits for stores of the same item.  Input buffers are never the
 // buffers results are stored from.  N > 0 is the compile-time factor size
 // (thread gram rows in registers for N <= 6); N == 0 handles any n.
-template <typename T, int N, int PHASE, int NFC = 0>
+template <typename T, int N, int PHASE, int NFC, Layout LAY>
 __global__ void __launch_bounds__(256)
-kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *__restrict__ dY,
-                    T *__restrict__ dX, T *__restrict__ dA, T *__restrict__ W, AdjShape sh,
-                    int batch)
+kron_adjoint_kernel(AdjIO<T, LAY> io, T *__restrict__ W, AdjShape sh, int batch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
     constexpr bool KEEP = N > 0 && N <= 6;  // gram rows kept across an entry's chunks
@@ -242,7 +240,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
         int s0, Cx, cnt;
         geom(ck, s0, Cx, cnt);
         const long long eb = (long long)b * E;
-        const auto gdy = to_global(dY + eb);
+        const auto gdy = io.dy(b);
         if constexpr (PHASE == 1) {
             const int sb = (int)mul24(s0, (int)EF);
 #pragma unroll
@@ -251,7 +249,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
                 if (q < cnt) v0[k] = gdy[sb + q];
             }
         } else if constexpr (PHASE == 2) {
-            const auto gx = to_global(X + eb);
+            const auto gx = io.x(b);
             const auto gv = to_global(W + 2 * eb + ES * s0);
             const FastDiv dc = fast_div(Cx);
 #pragma unroll
@@ -349,10 +347,10 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             contract_fibers<T, T, TR, 8>(src, dst, outer, n, inner, n, fac(k), n);
         else if constexpr (N * N * sizeof(T) <= 288)
             // the factor through scalar loads (SGPRs): no LDS reads for it
-            adj_contract<T, N, TR>(src, dst, outer, inner,
-                                   to_const(A + (long long)cur_b * ddn + k * nn));
+            adj_contract<T, N, TR>(src, dst, outer, inner, to_const(io.fac(cur_b, k)),
+                                   io.lda());
         else
-            adj_contract<T, N, TR>(src, dst, outer, inner, fac(k));
+            adj_contract<T, N, TR>(src, dst, outer, inner, fac(k), N);
     };
     // this entry's share of dA: registers -> LDS -> one atomic per element
     auto flush = [&](int b) __attribute__((always_inline)) {
@@ -371,9 +369,8 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             }
         }
         __syncthreads();
-        const auto dAb = to_global(dA + (long long)b * ddn);
         for (int w = k0 * nn + tid; w < (k0 + ng) * nn; w += nthr) {
-            atomic_add(dAb + w, T(dAacc[w]));
+            io.add_da(b, w, T(dAacc[w]));
             dAacc[w] = 0;
         }
     };
@@ -388,7 +385,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
     for (long long r = r0; r < r1; ++r) {
         if (b != cur_b) {
             if (cur_b >= 0 && PHASE != 1) flush(cur_b);
-            for (int w = tid; w < ddn; w += nthr) facL[w] = A[(long long)b * ddn + w];
+            for (int w = tid; w < ddn; w += nthr) facL[w] = io.fac_elem(b, w);
             cur_b = b;
         }
         if (!staged) stage(ck);
@@ -490,7 +487,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
             }
         } else if constexpr (PHASE == 2) {
             const auto gu = to_global(W + 2 * eb + E + ES * s0);
-            const auto gx = to_global(dX + eb);
+            const auto gx = io.dx(b);
             const T *bu = B(h + 1);
             const FastDiv dc = fast_div(Cx);
 #pragma unroll
@@ -499,7 +496,7 @@ kron_adjoint_kernel(const T *__restrict__ A, const T *__restrict__ X, const T *_
                 if (q < cnt) {
                     if (nf > 0) gu[q] = bu[q];
                     const int is = quot(q, dc);
-                    gx[(int)mul24(is, (int)EF) + s0 + (q - (int)mul24(is, Cx))] = res[q];
+                    io.put_dx(gx, (int)mul24(is, (int)EF) + s0 + (q - (int)mul24(is, Cx)), res[q]);
                 }
             }
         }
@@ -659,56 +656,58 @@ size_t adjoint_scratch_elems(int d, int n)
     return (sh.h < d) ? (size_t)(2 * sh.E) : 0;
 }
 
-template <typename T, int N, int P>
+template <typename T, int N, int P, Layout LAY>
 static const void *adjoint_kernel_nf(int nf)
 {
     // compile-time fast-mode counts for the common split (passes 1 and 3)
     if constexpr (P != 2 && N >= 4 && N <= 6) {
         switch (nf) {
-        case 2: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 2>);
-        case 3: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 3>);
-        case 4: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 4>);
+        case 2: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 2, LAY>);
+        case 3: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 3, LAY>);
+        case 4: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 4, LAY>);
         default: break;
         }
     }
-    return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P>);
+    return reinterpret_cast<const void *>(kron_adjoint_kernel<T, N, P, 0, LAY>);
 }
 
-template <typename T, int P>
+template <typename T, int P, Layout LAY>
 static const void *adjoint_kernel_for(int n, int nf)
 {
     switch (n) {
-    case 2: return adjoint_kernel_nf<T, 2, P>(nf);
-    case 3: return adjoint_kernel_nf<T, 3, P>(nf);
-    case 4: return adjoint_kernel_nf<T, 4, P>(nf);
-    case 5: return adjoint_kernel_nf<T, 5, P>(nf);
-    case 6: return adjoint_kernel_nf<T, 6, P>(nf);
-    case 7: return adjoint_kernel_nf<T, 7, P>(nf);
-    case 8: return adjoint_kernel_nf<T, 8, P>(nf);
-    default: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 0, P>);
+    case 2: return adjoint_kernel_nf<T, 2, P, LAY>(nf);
+    case 3: return adjoint_kernel_nf<T, 3, P, LAY>(nf);
+    case 4: return adjoint_kernel_nf<T, 4, P, LAY>(nf);
+    case 5: return adjoint_kernel_nf<T, 5, P, LAY>(nf);
+    case 6: return adjoint_kernel_nf<T, 6, P, LAY>(nf);
+    case 7: return adjoint_kernel_nf<T, 7, P, LAY>(nf);
+    case 8: return adjoint_kernel_nf<T, 8, P, LAY>(nf);
+    default: return reinterpret_cast<const void *>(kron_adjoint_kernel<T, 0, P, 0, LAY>);
     }
 }
 
-template <typename T>
-kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T *dX, T *dA, T *W,
-                           int batch, hipStream_t stream)
+template <typename T, Layout LAY>
+static kron_status launch_adjoint_io(const AdjointArgs<T> &a, const AdjIO<T, LAY> &io)
 {
-    if (batch == 0) return KRON_OK;
+    const int d = a.d, n = a.n, batch = a.batch;
+    T *W = a.W;
     AdjShape sh;
     if (!adjoint_shape(d, n, (int)sizeof(T), sh)) return KRON_ENOTSUP;
     if (sh.h < d && !W) return KRON_EWORKSPACE;
     const int nf = d - sh.h;
-    // dA is accumulated atomically by the work items: clear it first (a fill
-    // kernel, so the sequence stays valid under stream capture)
-    const long long nA = (long long)batch * d * n * n;
-    hipLaunchKernelGGL(kron_fill_zero<T>, dim3((unsigned)std::min<long long>((nA + 255) / 256, 4096)),
-                       dim3(256), 0, stream, dA, nA);
+    if constexpr (!AdjIO<T, LAY>::kAccumulate) {
+        // dA is accumulated atomically by the work items: clear it first (a
+        // fill kernel, so the sequence stays valid under stream capture)
+        const long long nA = (long long)batch * d * n * n;
+        hipLaunchKernelGGL(kron_fill_zero<T>,
+                           dim3((unsigned)std::min<long long>((nA + 255) / 256, 4096)), dim3(256),
+                           0, a.stream, a.dA, nA);
+    }
     const long long fixed = adj_fixed_bytes(d, n, (int)sizeof(T));
     // d = 6, n = 6: compile-time kernels for the three passes
     if (d == 6 && n == 6 && (long long)batch * 36 <= 0x7fffffffLL) {
         for (int phase = 1; phase <= 3; ++phase) {
-            const kron_status st = launch_adjoint_rows6<T>(phase, A, X, dY, dX, dA, W, batch,
-                                                           stream);
+            const kron_status st = launch_adjoint_rows6<T>(phase, a);
             if (st != KRON_OK) return st;
         }
         return KRON_OK;
@@ -719,9 +718,9 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
                                 : phase == 2 ? sh.ES * sh.C2
                                              : (long long)sh.C3 * sh.EF;
         const size_t lds = (size_t)(fixed + adj_nbuf(phase, sh.h, nf) * chunk * (long long)sizeof(T));
-        const void *kern = phase == 1   ? adjoint_kernel_for<T, 1>(n, nf)
-                           : phase == 2 ? adjoint_kernel_for<T, 2>(n, nf)
-                                        : adjoint_kernel_for<T, 3>(n, nf);
+        const void *kern = phase == 1   ? adjoint_kernel_for<T, 1, LAY>(n, nf)
+                           : phase == 2 ? adjoint_kernel_for<T, 2, LAY>(n, nf)
+                                        : adjoint_kernel_for<T, 3, LAY>(n, nf);
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                (int)lds);
@@ -732,9 +731,11 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
         const long long nck = phase == 2 ? sh.nck2 : (sh.ES + (phase == 1 ? sh.C1 : sh.C3) - 1) /
                                                          (phase == 1 ? sh.C1 : sh.C3);
         const long long grid = std::max(1LL, std::min((long long)batch * nck, cap));
-        void *args[] = {(void *)&A,  (void *)&X, (void *)&dY, (void *)&dX,
-                        (void *)&dA, (void *)&W, (void *)&sh, (void *)&batch};
-        hipError_t le = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, lds, stream);
+        AdjIO<T, LAY> iov = io;
+        AdjShape shv = sh;
+        int bv = batch;
+        void *args[] = {(void *)&iov, (void *)&W, (void *)&shv, (void *)&bv};
+        hipError_t le = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, lds, a.stream);
         if (le != hipSuccess) { set_last_hip_error((int)le); return KRON_EHIP; }
     }
     hipError_t e = hipGetLastError();
@@ -742,11 +743,16 @@ kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T 
     return KRON_OK;
 }
 
-template kron_status launch_adjoint<double>(int, int, const double *, const double *,
-                                            const double *, double *, double *, double *, int,
-                                            hipStream_t);
-template kron_status launch_adjoint<float>(int, int, const float *, const float *, const float *,
-                                           float *, float *, float *, int, hipStream_t);
+template <typename T>
+kron_status launch_adjoint(const AdjointArgs<T> &a)
+{
+    if (a.batch == 0) return KRON_OK;
+    if (a.layout == Layout::Batched) return launch_adjoint_io<T>(a, batched_io(a));
+    return launch_adjoint_io<T>(a, ptr_io(a));
+}
+
+template kron_status launch_adjoint<double>(const AdjointArgs<double> &);
+template kron_status launch_adjoint<float>(const AdjointArgs<float> &);
 template size_t adjoint_scratch_elems<double>(int, int);
 template size_t adjoint_scratch_elems<float>(int, int);
 

@@ -21,6 +21,7 @@
 //           of 36 columns (kron_adj6_cols).
 #include <algorithm>
 
+#include "kron_adjoint.hpp"
 #include "kron_split.hpp"
 
 namespace kron {
@@ -42,20 +43,106 @@ __device__ __forceinline__ int a6_chunk_off(int r, int c, int C2)
 // The address goes through an empty asm, so a factor read again later (the
 // passes use A4 and A5 twice) is loaded again instead of being kept in
 // SGPRs from its first use.
-template <typename T, bool TRANS>
-__device__ __forceinline__ void a6_factor(T (&a)[kA6N][kA6N], const T *A, int b, int m)
+// The batched layout's operands as the d6n6 kernels take them: base
+// pointers only, every stride a compile-time constant (runtime ones, and
+// unused fields, cost SGPRs that the factors need).
+template <typename T>
+struct A6Batched {
+    const T *A, *X, *dY;
+    T *dX, *dA;
+    static constexpr bool kAccumulate = false;
+    __device__ __forceinline__ static void put_dx(gptr<T> p, int i, T v) { p[i] = v; }
+};
+template <typename T, Layout LAY>
+using A6IO = std::conditional_t<LAY == Layout::Batched, A6Batched<T>, AdjIO<T, Layout::Ptr>>;
+
+// Per-entry addresses (IO: A6Batched or AdjIO<T, Ptr>).
+template <typename IO>
+__device__ __forceinline__ auto a6_fac(const IO &io, int b, int m)
 {
-    const T *p = A + ((long long)b * 6 + m) * 36;
+    if constexpr (!IO::kAccumulate)
+        return io.A + ((long long)b * 6 + m) * 36;
+    else
+        return io.fac(b, m);
+}
+template <typename IO>
+__device__ __forceinline__ auto a6_x(const IO &io, int b)
+{
+    if constexpr (!IO::kAccumulate)
+        return to_global(io.X + (long long)b * kA6E);
+    else
+        return io.x(b);
+}
+template <typename IO>
+__device__ __forceinline__ auto a6_dy(const IO &io, int b)
+{
+    if constexpr (!IO::kAccumulate)
+        return to_global(io.dY + (long long)b * kA6E);
+    else
+        return io.dy(b);
+}
+template <typename IO>
+__device__ __forceinline__ auto a6_dx(const IO &io, int b)
+{
+    if constexpr (!IO::kAccumulate)
+        return to_global(io.dX + (long long)b * kA6E);
+    else
+        return io.dx(b);
+}
+// dA_k(i, j) += v, w = 36 k + i + 6 j
+template <typename IO, typename T>
+__device__ __forceinline__ void a6_add_da(const IO &io, int b, int w, T v)
+{
+    if constexpr (!IO::kAccumulate)
+        atomic_add(to_global(io.dA + (long long)b * 216 + w), v);
+    else
+        io.add_da(b, w, v);
+}
+
+template <typename T, bool TRANS, typename IO>
+__device__ __forceinline__ void a6_factor(T (&a)[kA6N][kA6N], const IO &io, int b, int m)
+{
+    const T *p = a6_fac(io, b, m);
     asm volatile("" : "+s"(p));
-    load_factor<T, T, kA6N, TRANS, true>(a, to_const(p), kA6N);
+    if constexpr (IO::kAccumulate) {
+        if (io.lda() != kA6N) {
+            load_factor<T, T, kA6N, TRANS, true>(a, to_const(p), io.lda());
+            return;
+        }
+    }
+    // 36 contiguous elements as 64-byte scalar loads (s_load_dwordx16; left
+    // to itself the compiler splits them into many narrow loads, which
+    // lengthens every workgroup's wait for its factor).  The vector types
+    // keep the element's alignment.
+    constexpr int V = 64 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V), aligned(sizeof(T))));
+    typedef T vec4 __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+    T f[kA6R];
+    const cptr<vec> pv = (cptr<vec>)to_const(p);
+#pragma unroll
+    for (int c = 0; c < kA6R / V; ++c) {
+        const vec x = pv[c];
+#pragma unroll
+        for (int e = 0; e < V; ++e) f[c * V + e] = x[e];
+    }
+#pragma unroll
+    for (int c = kA6R / V * V; c < kA6R; c += 4) {
+        const vec4 x = ((cptr<vec4>)to_const(p + c))[0];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f[c + e] = x[e];
+    }
+#pragma unroll
+    for (int j = 0; j < kA6N; ++j)
+#pragma unroll
+        for (int i = 0; i < kA6N; ++i) a[i][j] = TRANS ? f[j + i * kA6N] : f[i + j * kA6N];
 }
 
 }  // namespace
 
 // Pass 1: workgroup (entry blockIdx.x / 36, row blockIdx.x % 36).
-template <typename T>
+template <typename T, Layout LAY>
 __global__ void __launch_bounds__(256)
-kron_adj6_rows_v(const T *__restrict__ A, const T *__restrict__ dY, T *__restrict__ W, int C2)
+kron_adj6_rows_v(A6IO<T, LAY> io, T *__restrict__ W, int C2)
 {
     __shared__ T row[kA6C];
     // Workgroups are dealt to the 8 XCDs round robin; a contiguous range of
@@ -64,7 +151,7 @@ kron_adj6_rows_v(const T *__restrict__ A, const T *__restrict__ dY, T *__restric
     int bid = blockIdx.x;
     if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
     const int b = bid / kA6R, r = bid % kA6R, tid = threadIdx.x;
-    const auto g = to_global(dY + (long long)b * kA6E + r * kA6C);
+    const auto g = a6_dy(io, b) + r * kA6C;
     T v[kA6Per];
 #pragma unroll
     for (int k = 0; k < kA6Per; ++k) {
@@ -78,16 +165,16 @@ kron_adj6_rows_v(const T *__restrict__ A, const T *__restrict__ dY, T *__restric
     }
     __syncthreads();
     T a[kA6N][kA6N];
-    a6_factor<T, true>(a, A, b, 2);
+    a6_factor<T, true>(a, io, b, 2);
     split_contract_lds<T, T, kA6N, 216, kA6C>(row, a);
     __syncthreads();
-    a6_factor<T, true>(a, A, b, 3);
+    a6_factor<T, true>(a, io, b, 3);
     split_contract_lds<T, T, kA6N, 36, kA6C>(row, a);
     __syncthreads();
-    a6_factor<T, true>(a, A, b, 4);
+    a6_factor<T, true>(a, io, b, 4);
     split_contract_lds<T, T, kA6N, 6, kA6C>(row, a);
     __syncthreads();
-    a6_factor<T, true>(a, A, b, 5);
+    a6_factor<T, true>(a, io, b, 5);
     split_contract_lds<T, T, kA6N, 1, kA6C>(row, a);
     __syncthreads();
     const auto V = to_global(W + 2LL * kA6E * b);
@@ -181,10 +268,9 @@ __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
 //   restage   B0 = U, B1 = dY again, from the registers they were staged from
 //   grams     dA_3 (G3, P3), dA_6 (G6, P6)
 // The next item's rows are loaded into those registers after the restage.
-template <typename T>
+template <typename T, Layout LAY>
 __global__ void __launch_bounds__(256)
-kron_adj6_rows_grams(const T *__restrict__ A, const T *__restrict__ dY, T *__restrict__ dA,
-                     const T *__restrict__ W, int C2, int batch)
+kron_adj6_rows_grams(A6IO<T, LAY> io, const T *__restrict__ W, int C2, int batch)
 {
     __shared__ T bufs[4][kA6C];
     __shared__ double dAs[4 * 36];
@@ -204,7 +290,7 @@ kron_adj6_rows_grams(const T *__restrict__ A, const T *__restrict__ dY, T *__res
     T vu[kA6Per], vy[kA6Per];
     auto load = [&](int b, int r) {
         const auto u = to_global(W + 2LL * kA6E * b + kA6E);
-        const auto y = to_global(dY + (long long)b * kA6E + r * kA6C);
+        const auto y = a6_dy(io, b) + r * kA6C;
 #pragma unroll
         for (int k = 0; k < kA6Per; ++k) {
             const int i = tid + k * 256;
@@ -239,7 +325,7 @@ kron_adj6_rows_grams(const T *__restrict__ A, const T *__restrict__ dY, T *__res
             }
         __syncthreads();
         if (tid < 4 * 36) {
-            atomic_add(to_global(dA + ((long long)b * 6 + 2) * 36 + tid), T(dAs[tid]));
+            a6_add_da(io, b, 2 * 36 + tid, T(dAs[tid]));
             dAs[tid] = 0;
         }
     };
@@ -256,25 +342,25 @@ kron_adj6_rows_grams(const T *__restrict__ A, const T *__restrict__ dY, T *__res
         T a[kA6N][kA6N];
         // One factor (72 SGPRs in FP64) at a time: the scheduling barriers
         // keep the next factor's scalar loads behind the current contraction.
-        a6_factor<T, false>(a, A, b, 5);
+        a6_factor<T, false>(a, io, b, 5);
         a6_contract<T, 1>(bufs[0], bufs[2], a);  // P5
         __builtin_amdgcn_sched_barrier(0);
-        a6_factor<T, true>(a, A, b, 2);
+        a6_factor<T, true>(a, io, b, 2);
         a6_contract<T, 216>(bufs[1], bufs[3], a);  // G4
         __syncthreads();
-        a6_factor<T, false>(a, A, b, 4);
+        a6_factor<T, false>(a, io, b, 4);
         a6_contract<T, 6>(bufs[2], bufs[0], a);  // P4
         __builtin_amdgcn_sched_barrier(0);
-        a6_factor<T, true>(a, A, b, 3);
+        a6_factor<T, true>(a, io, b, 3);
         a6_contract<T, 36>(bufs[3], bufs[1], a);  // G5
         __syncthreads();
         if (pair == 0) a6_gram_rows<T, 36>(bufs[3], bufs[0], p0, acc[0]);  // dA_4
         else a6_gram_rows<T, 6>(bufs[1], bufs[2], p0, acc[0]);            // dA_5
         __syncthreads();
-        a6_factor<T, false>(a, A, b, 3);
+        a6_factor<T, false>(a, io, b, 3);
         a6_contract<T, 36>(bufs[0], bufs[2], a);  // P3
         __builtin_amdgcn_sched_barrier(0);
-        a6_factor<T, true>(a, A, b, 4);
+        a6_factor<T, true>(a, io, b, 4);
         a6_contract<T, 6>(bufs[1], bufs[3], a);  // G6
         __syncthreads();
         stage();
@@ -298,10 +384,9 @@ kron_adj6_rows_grams(const T *__restrict__ A, const T *__restrict__ dY, T *__res
 // A chunk in LDS is 36 x 36 row-major: mode 1 has stride 216, mode 2 stride
 // 36, as modes 3 and 4 of a row.  Waves 0, 1 accumulate dA_1, waves 2, 3 dA_2.
 // LDS: B0 = X, B1 = V, B2 = P1 (then U), B3 = G2 (then dX).
-template <typename T>
+template <typename T, Layout LAY>
 __global__ void __launch_bounds__(256)
-kron_adj6_cols(const T *__restrict__ A, const T *__restrict__ X, T *__restrict__ dX,
-               T *__restrict__ dA, T *__restrict__ W, int batch)
+kron_adj6_cols(A6IO<T, LAY> io, T *__restrict__ W, int batch)
 {
     constexpr int CW = 36;  // chunk width (columns)
     __shared__ T bufs[4][kA6C];
@@ -319,7 +404,7 @@ kron_adj6_cols(const T *__restrict__ A, const T *__restrict__ X, T *__restrict__
     T vx[kA6Per], vv[kA6Per];
     // element i of a chunk: row i / 36, column i % 36 of the chunk
     auto load = [&](int b, int k) {
-        const auto x = to_global(X + (long long)b * kA6E + k * CW);
+        const auto x = a6_x(io, b) + k * CW;
         const auto v = to_global(W + 2LL * kA6E * b + k * kA6C);
 #pragma unroll
         for (int q = 0; q < kA6Per; ++q) {
@@ -341,7 +426,7 @@ kron_adj6_cols(const T *__restrict__ A, const T *__restrict__ X, T *__restrict__
             }
         __syncthreads();
         if (tid < 2 * 36) {
-            atomic_add(to_global(dA + (long long)b * 216 + tid), T(dAs[tid]));
+            a6_add_da(io, b, tid, T(dAs[tid]));
             dAs[tid] = 0;
         }
     };
@@ -364,30 +449,30 @@ kron_adj6_cols(const T *__restrict__ A, const T *__restrict__ X, T *__restrict__
         }
         if (it + 1 < r1) load(nb, nk);
         T a[kA6N][kA6N];
-        a6_factor<T, false>(a, A, b, 1);
+        a6_factor<T, false>(a, io, b, 1);
         a6_contract<T, 36>(bufs[0], bufs[2], a);  // P1: mode 2 of X
         __builtin_amdgcn_sched_barrier(0);
-        a6_factor<T, true>(a, A, b, 0);
+        a6_factor<T, true>(a, io, b, 0);
         a6_contract<T, 216>(bufs[1], bufs[3], a);  // G2: mode 1 of V
         __syncthreads();
         if (pair == 0) a6_gram<T, 216>(bufs[1], bufs[2], half, acc);  // dA_1
         else a6_gram<T, 36>(bufs[3], bufs[0], half, acc);             // dA_2
         __syncthreads();
-        a6_factor<T, false>(a, A, b, 0);
+        a6_factor<T, false>(a, io, b, 0);
         a6_contract<T, 216>(bufs[2], bufs[2], a);  // U: mode 1 of P1
         __builtin_amdgcn_sched_barrier(0);
-        a6_factor<T, true>(a, A, b, 1);
+        a6_factor<T, true>(a, io, b, 1);
         a6_contract<T, 36>(bufs[3], bufs[3], a);  // dX: mode 2 of G2
         __syncthreads();
         const auto u = to_global(W + 2LL * kA6E * b + kA6E + k * kA6C);
-        const auto dx = to_global(dX + (long long)b * kA6E + k * CW);
+        const auto dx = a6_dx(io, b) + k * CW;
 #pragma unroll
         for (int q = 0; q < kA6Per; ++q) {
             const int i = tid + q * 256;
             if (i < kA6C) {
                 const int r = i / CW;
                 u[i] = bufs[2][i];
-                dx[r * kA6C + (i - r * CW)] = bufs[3][i];
+                io.put_dx(dx, r * kA6C + (i - r * CW), bufs[3][i]);
             }
         }
         if (it + 1 == r1 || nb != b) flush(b);
@@ -396,27 +481,28 @@ kron_adj6_cols(const T *__restrict__ A, const T *__restrict__ X, T *__restrict__
     }
 }
 
-template <typename T>
-kron_status launch_adjoint_rows6(int pass, const T *A, const T *X, const T *dY, T *dX, T *dA,
-                                 T *W, int batch, hipStream_t stream)
+template <typename T, Layout LAY>
+static kron_status launch_rows6_io(int pass, const AdjointArgs<T> &a, const A6IO<T, LAY> &io)
 {
     constexpr int C2 = 36;  // column-chunk width of the scratch layout
+    const int batch = a.batch;
     if ((long long)batch * kA6R > 0x7fffffffLL) return KRON_ENOTSUP;
     if (pass == 1) {
-        hipLaunchKernelGGL(kron_adj6_rows_v<T>, dim3((unsigned)(batch * kA6R)), dim3(256), 0,
-                           stream, A, dY, W, C2);
+        hipLaunchKernelGGL((kron_adj6_rows_v<T, LAY>), dim3((unsigned)(batch * kA6R)), dim3(256),
+                           0, a.stream, io, a.W, C2);
     } else {
-        const void *kern = pass == 2 ? reinterpret_cast<const void *>(kron_adj6_cols<T>)
-                                     : reinterpret_cast<const void *>(kron_adj6_rows_grams<T>);
+        const void *kern = pass == 2
+                               ? reinterpret_cast<const void *>(kron_adj6_cols<T, LAY>)
+                               : reinterpret_cast<const void *>(kron_adj6_rows_grams<T, LAY>);
         const long long resident = persistent_blocks(kern, 0);
         const long long grid =
             std::max(1LL, std::min((long long)batch * kA6R, resident > 0 ? resident : 512));
         if (pass == 2)
-            hipLaunchKernelGGL(kron_adj6_cols<T>, dim3((unsigned)grid), dim3(256), 0, stream, A,
-                               X, dX, dA, W, batch);
+            hipLaunchKernelGGL((kron_adj6_cols<T, LAY>), dim3((unsigned)grid), dim3(256), 0,
+                               a.stream, io, a.W, batch);
         else
-            hipLaunchKernelGGL(kron_adj6_rows_grams<T>, dim3((unsigned)grid), dim3(256), 0,
-                               stream, A, dY, dA, W, C2, batch);
+            hipLaunchKernelGGL((kron_adj6_rows_grams<T, LAY>), dim3((unsigned)grid), dim3(256), 0,
+                               a.stream, io, a.W, C2, batch);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -426,10 +512,16 @@ kron_status launch_adjoint_rows6(int pass, const T *A, const T *X, const T *dY, 
     return KRON_OK;
 }
 
-template kron_status launch_adjoint_rows6<double>(int, const double *, const double *,
-                                                  const double *, double *, double *, double *,
-                                                  int, hipStream_t);
-template kron_status launch_adjoint_rows6<float>(int, const float *, const float *, const float *,
-                                                 float *, float *, float *, int, hipStream_t);
+template <typename T>
+kron_status launch_adjoint_rows6(int pass, const AdjointArgs<T> &a)
+{
+    if (a.layout == Layout::Batched)
+        return launch_rows6_io<T, Layout::Batched>(pass, a,
+                                                   A6Batched<T>{a.A, a.X, a.dY, a.dX, a.dA});
+    return launch_rows6_io<T, Layout::Ptr>(pass, a, ptr_io(a));
+}
+
+template kron_status launch_adjoint_rows6<double>(int, const AdjointArgs<double> &);
+template kron_status launch_adjoint_rows6<float>(int, const AdjointArgs<float> &);
 
 }  // namespace kron

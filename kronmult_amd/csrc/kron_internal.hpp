@@ -152,21 +152,37 @@ void *stream_workspace_slot(hipStream_t stream, size_t bytes, int slot, bool *fr
 template <typename T>
 const T *const *factor_table(const T *A, long long fstride, long long count, hipStream_t s);
 
-// Adjoint of the batched layout (kron_adjoint.hip): dX = K^T dY and dA_i,
-// both overwritten.  W: 2*n^d elements per entry (or nullptr when
-// adjoint_scratch_elems returns 0).
+// One adjoint call (kron_adjoint.hip): dX = K^T dY and every dA_k per entry.
+//   Batched: A (n,n,d,batch), X/dY/dX (n^d,batch), dA (n,n,d,batch); dX and
+//            dA are overwritten.
+//   Ptr:     factor k of entry b at pA[b*d+k] (leading dimension lda), X and
+//            dY vectors at pX[b] and pdY[b]; K_b^T dY is added into pdX[b]
+//            and dA_k into pdA[b*d+k] (leading dimension ldda), atomically:
+//            entries may share any of these.
+// W: adjoint_scratch_elems(d, n) elements per entry, entry b's at
+// W + b * that (nullptr when it is 0).
 template <typename T>
-kron_status launch_adjoint(int d, int n, const T *A, const T *X, const T *dY, T *dX, T *dA, T *W,
-                           int batch, hipStream_t stream);
+struct AdjointArgs {
+    Layout layout = Layout::Batched;
+    int d = 0, n = 0, batch = 0;
+    const T *A = nullptr, *X = nullptr, *dY = nullptr;
+    T *dX = nullptr, *dA = nullptr;
+    const T *const *pA = nullptr, *const *pX = nullptr, *const *pdY = nullptr;
+    T *const *pdX = nullptr, *const *pdA = nullptr;
+    int lda = 0, ldda = 0;
+    T *W = nullptr;
+    hipStream_t stream = nullptr;
+};
+template <typename T>
+kron_status launch_adjoint(const AdjointArgs<T> &a);
 template <typename T>
 size_t adjoint_scratch_elems(int d, int n);
 // The d = 6, n = 6 adjoint (kron_adjoint_rows.hip), pass 1 (V into W's
 // first vector of each entry), 2 (U, dX, dA_1, dA_2) or 3 (dA_3..dA_6); the
-// dA sums are added into dA (cleared beforehand).  KRON_ENOTSUP when
-// batchCount * 36 exceeds the int range.
+// dA sums are added into dA (cleared beforehand in the batched layout).
+// KRON_ENOTSUP when batchCount * 36 exceeds the int range.
 template <typename T>
-kron_status launch_adjoint_rows6(int pass, const T *A, const T *X, const T *dY, T *dX, T *dA,
-                                 T *W, int batch, hipStream_t stream);
+kron_status launch_adjoint_rows6(int pass, const AdjointArgs<T> &a);
 
 // The reference's per-entry vbatched structure (kron_plain.hip): one
 // workgroup per entry, every mode a pass through the entry's scratch slot of

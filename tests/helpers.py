@@ -151,3 +151,36 @@ def mixed_class_counts(total, seed, d_lo=2, d_hi=6, n_lo=3, n_hi=8, max_elems=65
             got += 1
         stream += 1
     return dict(sorted(out.items()))
+
+
+def modes(F, t, skip=None):
+    """Apply F[:, k] (batch, n, n) along mode k of t (batch, n, ..., n), k != skip."""
+    d = F.shape[1]
+    for k in range(d):
+        if k == skip:
+            continue
+        tk = np.moveaxis(t, k + 1, 1)
+        shp = tk.shape
+        out = np.matmul(F[:, k], tk.reshape(shp[0], shp[1], -1))
+        t = np.moveaxis(out.reshape(shp), 1, k + 1)
+    return t
+
+
+def grads_batched(d, n, A, X, dY):
+    """Per-mode formula, vectorised over entries: dX = kron(A_k^T) dY;
+    dA_k(p,q) = sum over the other indices of dY[..p..] * U_k[..q..], with
+    U_k = X contracted by every factor except k.  A is (batch, d, n*n)
+    column-major; dA is returned in the same layout."""
+    batch = X.shape[0]
+    F = np.swapaxes(A.reshape(batch, d, n, n).astype(np.float64), -1, -2)  # F[b,k][i,j]
+    shape = (batch,) + (n,) * d
+    dX = modes(np.swapaxes(F, -1, -2), dY.reshape(shape).astype(np.float64)).reshape(batch, -1)
+    dA = np.zeros((batch, d, n * n))
+    Yt = dY.reshape(shape).astype(np.float64)
+    for k in range(d):
+        U = modes(F, X.reshape(shape).astype(np.float64), skip=k)
+        Yk = np.moveaxis(Yt, k + 1, 1).reshape(batch, n, -1)
+        Uk = np.moveaxis(U, k + 1, 1).reshape(batch, n, -1)
+        M = np.matmul(Yk, np.swapaxes(Uk, 1, 2))  # M[b][p, q]
+        dA[:, k] = np.swapaxes(M, 1, 2).reshape(batch, -1)  # (p, q) at p + q*n
+    return dX, dA

@@ -109,39 +109,6 @@ def test_cfg_large_n_2048_f32(d, n):
     assert H.norm_rel_err(got, want) < 2e-6
 
 
-def _modes(F, t, skip=None):
-    """Apply F[:, k] (batch, n, n) along mode k of t (batch, n, ..., n), k != skip."""
-    d = F.shape[1]
-    for k in range(d):
-        if k == skip:
-            continue
-        tk = np.moveaxis(t, k + 1, 1)
-        shp = tk.shape
-        out = np.matmul(F[:, k], tk.reshape(shp[0], shp[1], -1))
-        t = np.moveaxis(out.reshape(shp), 1, k + 1)
-    return t
-
-
-def _grads_batched(d, n, A, X, dY):
-    """Per-mode formula, vectorised over entries: dX = kron(A_k^T) dY;
-    dA_k(p,q) = sum over the other indices of dY[..p..] * U_k[..q..], with
-    U_k = X contracted by every factor except k.  A is (batch, d, n*n)
-    column-major; dA is returned in the same layout."""
-    batch = X.shape[0]
-    F = np.swapaxes(A.reshape(batch, d, n, n).astype(np.float64), -1, -2)  # F[b,k][i,j]
-    shape = (batch,) + (n,) * d
-    dX = _modes(np.swapaxes(F, -1, -2), dY.reshape(shape).astype(np.float64)).reshape(batch, -1)
-    dA = np.zeros((batch, d, n * n))
-    Yt = dY.reshape(shape).astype(np.float64)
-    for k in range(d):
-        U = _modes(F, X.reshape(shape).astype(np.float64), skip=k)
-        Yk = np.moveaxis(Yt, k + 1, 1).reshape(batch, n, -1)
-        Uk = np.moveaxis(U, k + 1, 1).reshape(batch, n, -1)
-        M = np.matmul(Yk, np.swapaxes(Uk, 1, 2))  # M[b][p, q]
-        dA[:, k] = np.swapaxes(M, 1, 2).reshape(batch, -1)  # (p, q) at p + q*n
-    return dX, dA
-
-
 def _rowdot(a, b):
     return np.einsum("ij,ij->i", a, b)
 
@@ -179,7 +146,7 @@ def test_cfg_fwd_adjoint_d6n6_4096():
 
     # dX and dA against the per-mode formula on a spread sample
     idx = np.unique(np.linspace(0, batch - 1, 32).astype(np.int64))
-    rdX, rdA = _grads_batched(d, n, A[idx], X[idx], dY[idx])
+    rdX, rdA = H.grads_batched(d, n, A[idx], X[idx], dY[idx])
     assert H.rel_err(dX[idx], rdX) < F64_TOL
     assert H.rel_err(dA[idx], rdA) < F64_TOL
 
@@ -273,7 +240,7 @@ def test_adjoint_walk_integer(d, n, batch):
     grid): integer inputs, so dX and dA are exact whatever the order."""
     A, X, dY = _int_batched(d, n, batch, 900 + 10 * d + n, -1, 1, -3, 3)
     dX, dA = _gpu_adjoint(d, n, A, X, dY)
-    rdX, rdA = _grads_batched(d, n, A, X, dY)
+    rdX, rdA = H.grads_batched(d, n, A, X, dY)
     assert np.array_equal(dX, rdX)
     assert np.array_equal(dA.reshape(rdA.shape), rdA)
 
@@ -286,7 +253,7 @@ def test_adjoint_walk_f32(d, n, batch):
     X = H.uniform(batch * n ** d, 32).reshape(batch, -1).astype(np.float32)
     dY = H.uniform(batch * n ** d, 33).reshape(batch, -1).astype(np.float32)
     dX, dA = _gpu_adjoint(d, n, A, X, dY)
-    rdX, rdA = _grads_batched(d, n, A, X, dY)
+    rdX, rdA = H.grads_batched(d, n, A, X, dY)
     # sums of n^(d-1) products for dA: relative to the gradients' scale
     assert H.norm_rel_err(dX, rdX) < 1e-5
     assert H.norm_rel_err(dA.reshape(rdA.shape), rdA) < 1e-5
