@@ -267,7 +267,8 @@ kron_status kron_xbatched_plan_destroy(kron_xplan plan);
  *   the result to the cache; without it uncached classes use the default
  *   mapping and a byte-count cost model.
  *   KRON_MIXED_ATOMIC forces atomic Y updates everywhere;
- *   KRON_MIXED_FORCE_V0 / _V1 force mapping 0 / 1 where it exists (tests).
+ *   KRON_MIXED_FORCE_V0 / _V1 / _V2 force mapping 0 / 1 / 2 where it exists
+ *   (tests).
  * The contents of A, X and Y may change between executes; the pointer
  * arrays must not.  plan_capture records `applies` executes in a hipGraph
  * (on a plan-owned stream); plan_replay launches it on `stream`.
@@ -293,12 +294,14 @@ typedef struct kron_mixed_class_f32 {
 #define KRON_MIXED_ATOMIC 0x2
 #define KRON_MIXED_FORCE_V0 0x100
 #define KRON_MIXED_FORCE_V1 0x200
+#define KRON_MIXED_FORCE_V2 0x400
 
 /* How a class of a mixed plan runs (kron_mixed_plan_class_info). */
 #define KRON_MIXED_KIND_LAUNCH 0   /* separate launch (outside the compiled set) */
 #define KRON_MIXED_KIND_ONCHIP 1   /* one entry per workgroup, entry in LDS */
 #define KRON_MIXED_KIND_PACKED 2   /* several entries per workgroup */
 #define KRON_MIXED_KIND_TWOPASS 3  /* slow / fast mode passes through scratch */
+#define KRON_MIXED_KIND_SLAB 4     /* n slabs of the first mode per entry, one item each */
 
 typedef struct kron_mplan_s *kron_mplan;
 kron_status kron_mixed_plan_create_f64(kron_mplan *plan, int nclass,
@@ -317,9 +320,11 @@ kron_status kron_mixed_plan_replay(kron_mplan plan, kron_stream_t stream);
  * of the whole GPU when tuned, else bytes); any output pointer may be NULL. */
 kron_status kron_mixed_plan_class_info(kron_mplan plan, int cls, int *kind, int *variant,
                                        int *atomic, double *cost);
-/* Work items, persistent workgroups and dynamic LDS bytes of the launch. */
+/* Work items, persistent workgroups and dynamic LDS bytes of the persistent
+ * kernel, and the kernel launches one execute makes (1 unless a class is
+ * forced onto the two-pass mapping or runs outside the compiled set). */
 kron_status kron_mixed_plan_launch_info(kron_mplan plan, int *items, int *grid,
-                                        int *lds_bytes);
+                                        int *lds_bytes, int *launches);
 kron_status kron_mixed_plan_destroy(kron_mplan plan);
 
 /*

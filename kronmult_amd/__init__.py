@@ -98,7 +98,7 @@ def lib():
                        ("replay", [_c_vp, _c_vp]), ("destroy", [_c_vp]),
                        ("class_info", [_c_vp, _c_int, _c_ip, _c_ip, _c_ip,
                                        ctypes.POINTER(ctypes.c_double)]),
-                       ("launch_info", [_c_vp, _c_ip, _c_ip, _c_ip])):
+                       ("launch_info", [_c_vp, _c_ip, _c_ip, _c_ip, _c_ip])):
         f = getattr(L, "kron_mixed_plan_" + name)
         f.argtypes = args
         f.restype = _c_int
@@ -298,8 +298,8 @@ class XbatchedPlan:
 
 
 KRON_MIXED_TUNE, KRON_MIXED_ATOMIC = 0x1, 0x2
-KRON_MIXED_FORCE_V0, KRON_MIXED_FORCE_V1 = 0x100, 0x200
-MIXED_KINDS = {0: "launch", 1: "onchip", 2: "packed", 3: "twopass"}
+KRON_MIXED_FORCE_V0, KRON_MIXED_FORCE_V1, KRON_MIXED_FORCE_V2 = 0x100, 0x200, 0x400
+MIXED_KINDS = {0: "launch", 1: "onchip", 2: "packed", 3: "twopass", 4: "slab"}
 
 
 class _CMixedClass(ctypes.Structure):
@@ -360,11 +360,12 @@ class MixedPlan:
                 "cost": cost.value}
 
     def launch_info(self):
-        items, grid, lds = _c_int(), _c_int(), _c_int()
+        items, grid, lds, nl = _c_int(), _c_int(), _c_int(), _c_int()
         _check("kron_mixed_plan_launch_info",
                lib().kron_mixed_plan_launch_info(self._h, ctypes.byref(items), ctypes.byref(grid),
-                                                 ctypes.byref(lds)))
-        return {"items": items.value, "grid": grid.value, "lds_bytes": lds.value}
+                                                 ctypes.byref(lds), ctypes.byref(nl)))
+        return {"items": items.value, "grid": grid.value, "lds_bytes": lds.value,
+                "launches": nl.value}
 
     def close(self):
         if self._h:

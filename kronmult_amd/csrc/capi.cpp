@@ -605,10 +605,11 @@ kron_status kron_mixed_plan_class_info(kron_mplan plan, int cls, int *kind, int 
     if (!plan) return KRON_EINVAL;
     return mixed_class_info(plan->core, cls, kind, variant, atomic, cost);
 }
-kron_status kron_mixed_plan_launch_info(kron_mplan plan, int *items, int *grid, int *lds_bytes)
+kron_status kron_mixed_plan_launch_info(kron_mplan plan, int *items, int *grid, int *lds_bytes,
+                                        int *launches)
 {
     if (!plan) return KRON_EINVAL;
-    return mixed_launch_info(plan->core, items, grid, lds_bytes);
+    return mixed_launch_info(plan->core, items, grid, lds_bytes, launches);
 }
 kron_status kron_mixed_plan_destroy(kron_mplan plan)
 {

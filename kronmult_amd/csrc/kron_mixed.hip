@@ -10,17 +10,18 @@
 //
 //   * items: a Y-target group of an on-chip class (its entries run one after
 //     another in one workgroup, so Y is updated with plain read-modify-write),
-//     P entries of a small class (packed into one workgroup, atomic Y), or
-//     one tile of a large class's slow-mode pass (X -> scratch) or fast-mode
-//     pass (scratch -> Y, for a Y-target group);
-//   * with large classes present an apply is two launches of the kernel:
-//     the first holds every slow-mode tile, the second every fast-mode tile,
-//     and the whole-entry items are split between them to balance the two;
+//     P entries of a small class (packed into one workgroup, atomic Y), or,
+//     for a class too large for LDS, S slabs of its first mode for a
+//     Y-target group (every slab reads X and adds its own rows of Y, so
+//     items never wait for one another and an apply is one launch);
 //   * the items are sorted by estimated cost, largest first, and taken from
 //     an atomic counter (longest-processing-time-first list scheduling);
 //   * each class runs a compile-time (D, N) path with a tile and thread
 //     mapping variant chosen by the autotuner (threads per entry for on-chip
-//     classes, the slow/fast mode split for large ones);
+//     classes, slabs per item for large ones);
+//   * a third mapping of the large classes, two passes through scratch
+//     (slow modes, then fast modes), needs the apply split into two
+//     launches; it is kept for comparison and never chosen by the tuner;
 //   * the counter resets itself (the last workgroup out clears it), so the
 //     launch needs no memset and replays inside a hipGraph.
 //
@@ -88,23 +89,33 @@ struct MixPath {
     static constexpr int F = E / N;
     static constexpr int G0 = F >= kMixThreads ? kMixThreads : pow2ceil(F);
     static constexpr bool BIG = E > L;
+    // large classes: variants 0 and 1 run an entry as N slabs of its first
+    // mode (no scratch) -- 0 with as many slabs per work item as LDS holds
+    // (fewer reads of X), 1 with one (more, smaller items) -- and variant 2
+    // in two passes through scratch
+    static constexpr int SMAX = BIG ? (L / F < N ? L / F : N) : 0;
+    static constexpr bool SLAB = BIG && V <= 1;
+    static constexpr int SLABS = V == 0 ? SMAX : 1;  // slabs per item
+    static constexpr bool TWOPASS = BIG && V >= 2;
     // on-chip classes: variant v halves the threads per entry v times
     static constexpr int G = BIG ? kMixThreads : (G0 >> V) > 0 ? (G0 >> V) : 1;
     static constexpr int P = kMixThreads / G;
     // elements of a two-pass tile
     static constexpr int LT = L;
-    static constexpr MixSplit SP = mix_split(D, N, LT, BIG ? V : 0);
-    static constexpr bool VALID =
-        BIG ? SP.cost >= 0 : ((G0 >> V) >= 1 && (size_t)P * E <= (size_t)L);
+    static constexpr MixSplit SP = mix_split(D, N, LT, TWOPASS ? V - 2 : 0);
+    static constexpr bool VALID = SLAB      ? SMAX >= 1 && (V == 0 || SMAX > 1)
+                                  : TWOPASS ? SP.cost >= 0
+                                            : ((G0 >> V) >= 1 && (size_t)P * E <= (size_t)L);
     static constexpr size_t lds_bytes()
     {
-        if (BIG) {
+        if (SLAB) return (size_t)SLABS * F * sizeof(T);
+        if (TWOPASS) {
             const int S = cpow(N, SP.h), Fs = cpow(N, D - SP.h);
             return (size_t)std::max(S * SP.c1, SP.c2 * Fs) * sizeof(T);
         }
         return D == 1 ? 0 : (size_t)P * E * sizeof(T);
     }
-    static constexpr long long scratch_elems() { return BIG ? E : 0; }
+    static constexpr long long scratch_elems() { return TWOPASS ? E : 0; }
 };
 
 // Largest factor (bytes) held in SGPRs: up to 6 x 6 in FP64.  The kernel keeps
@@ -130,11 +141,11 @@ struct MixCtx {
     const MixedItem *items;
     const MixedEntry *recs;  // per-entry pointers, in item order
     int nitems;
-    unsigned *queue;
+    unsigned *queue;   // the take counter, then the exit counter
     T *scratch;
     T *smem;
-    int *s_next;     // LDS slot broadcasting a dequeued index
-    unsigned ahead;  // thread 0: the index requested for the item after next
+    int *s_next;       // LDS slot broadcasting a dequeued index
+    unsigned ahead;    // thread 0: the index requested for the item after next
 };
 
 // The next item index (workgroup-uniform).  Thread 0 keeps one request in
@@ -755,14 +766,121 @@ struct PathPass2 {
     }
 };
 
-// The paths of (D, N, variant); chunk selects pass 1 (>= 0) or pass 2 (<= -2)
-// of two-pass classes.
+// A large entry as N slabs of its first mode: slab i of Y is
+//     Y[i * F + s] += ((A_2 (x) .. (x) A_D) Z_i)[s],  Z_i[s] = sum_j A_1(i, j) X[j * F + s],
+// F = N^(D-1).  A work item takes S consecutive slabs, i0 = its chunk:
+// Z_i0 .. Z_(i0+S-1) are formed straight from X into LDS (S FMAs per X
+// element read), modes D..3 are contracted there in place for all of them
+// and mode 2 goes into Y.  An entry's X is read ceil(N / S) times; the
+// entry's items are neighbours in the queue, so the re-reads find X in the
+// caches more often than not.  No scratch and no order between work items:
+// the classes that would need a second pass through scratch run in the same
+// launch as the rest.  The units of an item are the entries of its group
+// (they share the Y target when the class updates Y without atomics; each
+// thread updates the same Y elements every time).
+template <typename T, typename Acc, int D, int N, int S>
+struct PathSlab {
+    static constexpr bool PER_ENTRY = true;
+    static constexpr bool PIPE = false;
+    static constexpr int F = cpow(N, D - 1);
+    static constexpr int IT = (F + kMixThreads - 1) / kMixThreads;
+    // fiber iterations whose loads are in flight together (about 36 values)
+    static constexpr int B = 36 / N > 0 ? 36 / N : 1;
+    static constexpr int NB = (IT + B - 1) / B;
+    static constexpr int S2 = cpow(N, D - 2);
+    struct Regs {};
+    static __device__ __forceinline__ T &L(const MixCtx<T> &x, int i)
+    {
+        return x.smem[lds_index<D - 1, N>(i)];
+    }
+    static __device__ __forceinline__ void load(Regs &, const MixCtx<T> &, const MixUnit<T> &,
+                                                int, int)
+    {
+    }
+    static __device__ __forceinline__ void first(const Regs &, const MixCtx<T> &,
+                                                 const MixUnit<T> &, int, int)
+    {
+    }
+    static __device__ __forceinline__ void direct(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                  int tid)
+    {
+        const int r = u.first + k, i0 = u.chunk;
+        const auto fp = mix_factor<T>(x, r, 0);
+        // rows i0 .. i0+S-1 of A_1; rows past N contribute zero slabs, which
+        // are not stored
+        Acc a1[S][N];
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                a1[q][j] = i0 + q < N ? Acc(fp[i0 + q + j * u.c.lda]) : Acc(0);
+        const auto xp = mix_x<T>(x, r);
+        T in[2][B][N];
+        auto fetch = [&](int bi, T(&dst)[B][N]) {
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int e = tid + (bi * B + q) * kMixThreads;
+                if (bi * B + q < IT && ((F % kMixThreads) == 0 || e < F)) {
+#pragma unroll
+                    for (int j = 0; j < N; ++j) dst[q][j] = xp[e + j * F];
+                }
+            }
+        };
+        fetch(0, in[0]);
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi) {
+            if (bi + 1 < NB) fetch(bi + 1, in[(bi + 1) & 1]);
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int e = tid + (bi * B + q) * kMixThreads;
+                if (bi * B + q < IT && ((F % kMixThreads) == 0 || e < F)) {
+#pragma unroll
+                    for (int sl = 0; sl < S; ++sl) {
+                        Acc v = a1[sl][0] * Acc(in[bi & 1][q][0]);
+#pragma unroll
+                        for (int j = 1; j < N; ++j)
+                            v = __builtin_fma(a1[sl][j], Acc(in[bi & 1][q][j]), v);
+                        L(x, sl * F + e) = T(v);
+                    }
+                }
+            }
+        }
+    }
+    static __device__ __forceinline__ void rest(const MixCtx<T> &x, const MixUnit<T> &u, int k,
+                                                int tid)
+    {
+        const int r = u.first + k, i0 = u.chunk;
+        const int nvalid = (N - i0 < S ? N - i0 : S) * F;
+        Acc a[N][N];
+        __syncthreads();
+        // original mode K is mode K - 1 of each slab's D - 1 modes; a pass
+        // covers the fibers of all S slabs
+        for_modes_desc<D, 3>([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, K - 1),
+                                                            u.c.lda);
+            tile_pass<Acc, N, cpow(N, D - K), S * S2, kMixThreads>(
+                tid, a, [&](int e) { return L(x, e); }, [&](int e, Acc v) { L(x, e) = T(v); });
+            __syncthreads();
+        });
+        load_factor<T, Acc, N, false, true, kMixSBytes>(a, mix_factor<T>(x, r, 1), u.c.lda);
+        tile_pass_y<T, Acc, N, S2, S * S2, kMixThreads>(
+            tid, a, [&](int e) { return L(x, e); }, mix_y<T>(x, r) + (long long)i0 * F,
+            [](int e) { return e; }, [&](int e) { return e < nvalid; }, u.c.atomic_y != 0);
+        __syncthreads();  // LDS is rewritten by the next unit
+    }
+};
+
+// The paths of (D, N, variant); chunk selects the slab of slab classes, and
+// pass 1 (>= 0) or pass 2 (<= -2) of two-pass classes.
 template <typename T, int D, int N, int V>
 __device__ __forceinline__ int mix_run(MixCtx<T> &x, int it, int code, int tid)
 {
     using Pth = MixPath<T, D, N, V>;
     if constexpr (!Pth::VALID) {
         return mix_dequeue(x);
+    } else if constexpr (Pth::SLAB) {
+        return mix_pipeline<T, PathSlab<T, T, D, N, Pth::SLABS>>(x, it, code, tid);
     } else if constexpr (Pth::BIG) {
         if (to_const(x.items)[it].chunk >= 0)
             return mix_pipeline<T, PathPass1<T, T, D, N, Pth::SP.h, Pth::SP.c1>>(x, it, code, tid);
@@ -806,7 +924,8 @@ kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const MixedE
         switch (code) {
 #define KRON_MIX_CASE_V(D, N, V)                                                              \
     case mix_code(D, N, V): it = mix_run<T, D, N, V>(x, it, code, tid); break;
-#define KRON_MIX_CASE(D, N) KRON_MIX_CASE_V(D, N, 0) KRON_MIX_CASE_V(D, N, 1)
+#define KRON_MIX_CASE(D, N) \
+    KRON_MIX_CASE_V(D, N, 0) KRON_MIX_CASE_V(D, N, 1) KRON_MIX_CASE_V(D, N, 2)
             KRON_MIX_FOR_EACH(KRON_MIX_CASE)
 #undef KRON_MIX_CASE
 #undef KRON_MIX_CASE_V
@@ -836,18 +955,22 @@ static bool path_info(int D, int N, int V, MixPathInfo &out)
     if (D == DD && N == NN && V == VV) {                                                      \
         using Pth = MixPath<T, DD, NN, VV>;                                                   \
         out.valid = Pth::VALID;                                                               \
-        out.big = Pth::BIG;                                                                   \
+        out.big = Pth::TWOPASS;                                                               \
+        out.slab = Pth::SLAB;                                                                 \
+        out.slabs = Pth::SLAB ? Pth::SLABS : 0;                                               \
         out.packed = !Pth::BIG && (Pth::P > 1 || DD == 1);                                    \
         out.per_item = Pth::BIG ? 1 : Pth::P;                                                 \
         out.lds = Pth::VALID ? Pth::lds_bytes() : 0;                                          \
         out.scratch = Pth::VALID ? Pth::scratch_elems() : 0;                                  \
-        out.split = Pth::BIG ? Pth::SP.h : 0;                                                 \
-        out.tiles1 = Pth::BIG ? (cpow(NN, DD - Pth::SP.h) + Pth::SP.c1 - 1) / Pth::SP.c1 : 0;  \
-        out.tiles2 = Pth::BIG ? (cpow(NN, Pth::SP.h) + Pth::SP.c2 - 1) / Pth::SP.c2 : 0;       \
+        out.split = Pth::TWOPASS ? Pth::SP.h : 0;                                             \
+        out.tiles1 = Pth::TWOPASS ? (cpow(NN, DD - Pth::SP.h) + Pth::SP.c1 - 1) / Pth::SP.c1   \
+                                  : Pth::SLAB ? (NN + Pth::SLABS - 1) / Pth::SLABS : 0;       \
+        out.tiles2 = Pth::TWOPASS ? (cpow(NN, Pth::SP.h) + Pth::SP.c2 - 1) / Pth::SP.c2 : 0;   \
         out.threads = Pth::G;                                                                 \
         return true;                                                                          \
     }
-#define KRON_MIX_INFO(DD, NN) KRON_MIX_INFO_V(DD, NN, 0) KRON_MIX_INFO_V(DD, NN, 1)
+#define KRON_MIX_INFO(DD, NN) \
+    KRON_MIX_INFO_V(DD, NN, 0) KRON_MIX_INFO_V(DD, NN, 1) KRON_MIX_INFO_V(DD, NN, 2)
     KRON_MIX_FOR_EACH(KRON_MIX_INFO)
 #undef KRON_MIX_INFO
 #undef KRON_MIX_INFO_V

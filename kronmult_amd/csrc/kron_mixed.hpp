@@ -11,7 +11,7 @@ namespace kron {
 
 constexpr int kMixThreads = 256;       // threads per workgroup
 constexpr int kMixLdsBytes = 65536;    // LDS of the largest path (2 workgroups per CU)
-constexpr int kMixVariants = 2;        // tile / thread mapping variants per (d, n)
+constexpr int kMixVariants = 3;        // tile / thread mapping variants per (d, n)
 
 // One shape class as the kernel reads it (device table).
 template <typename T>
@@ -40,19 +40,21 @@ struct MixedItem {
     int cls;
     int first;
     int count;
-    int chunk;  // -1: whole entries; c >= 0: pass-1 tile c; -2 - c: pass-2 tile c
+    int chunk;  // -1: whole entries; c >= 0: slab c, or pass-1 tile c; -2 - c: pass-2 tile c
 };
 
 // What a compiled (d, n, variant) path needs.
 struct MixPathInfo {
     bool valid = false;
-    bool big = false;       // two passes through per-workgroup scratch
+    bool big = false;       // two passes through per-entry scratch
+    bool slab = false;      // an entry as n slabs of its first mode (no scratch)
+    int slabs = 0;          // slabs per work item (slab paths)
     bool packed = false;    // several entries per workgroup, atomic Y
     int per_item = 1;       // entries per item when packed
     size_t lds = 0;         // dynamic LDS bytes
     long long scratch = 0;  // scratch elements per workgroup
     int split = 0;          // slow modes of pass 1 (big paths)
-    int tiles1 = 0;         // pass-1 tiles per entry (big paths)
+    int tiles1 = 0;         // pass-1 tiles (big paths) or slab items (slab paths) per entry
     int tiles2 = 0;         // pass-2 tiles per entry (big paths)
     int threads = 0;        // threads per entry
 };
@@ -79,7 +81,7 @@ kron_status mixed_capture(MixedPlanCore *p, int applies);
 kron_status mixed_replay(MixedPlanCore *p, hipStream_t s);
 kron_status mixed_class_info(MixedPlanCore *p, int cls, int *kind, int *variant, int *atomic,
                              double *cost);
-kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds);
+kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds, int *launches);
 void mixed_destroy(MixedPlanCore *p);
 
 template <typename T>

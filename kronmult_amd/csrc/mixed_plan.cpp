@@ -101,6 +101,8 @@ double model_cost_us(int d, int n, const MixPathInfo &info)
     const double E = (double)ipow_ll(n, d);
     double bytes = sizeof(T) * (3.0 * E + (double)d * n * n) + 8.0 * (d + 2);
     if (info.big) bytes += 2.0 * sizeof(T) * E;  // the scratch round trip
+    // slab classes read X n times, all but once mostly from the caches
+    if (info.slab) bytes += 0.25 * (n - 1) * sizeof(T) * E;
     return bytes / kModelBytesPerUs;
 }
 
@@ -237,11 +239,14 @@ static kron_status tune_class(int d, int n, hipStream_t stream, Tuned &best)
     for (int v = 0; v < kMixVariants && st == KRON_OK; ++v) {
         MixPathInfo info;
         if (!mixed_path_info<T>(d, n, v, info)) continue;
+        // two-pass mappings are not candidates: with one of them an apply
+        // takes two launches, and the slab mappings beat them
+        if (info.big) continue;
         void **pp = static_cast<void **>(ptrs);
         std::vector<ClassDesc> one(1);
         one[0] = ClassDesc{d, n, n, (int)nent, pp, pp + nent * d, pp + nent * (d + 1)};
         MixedPlanCore *sub = nullptr;
-        const int fl = v == 0 ? KRON_MIXED_FORCE_V0 : KRON_MIXED_FORCE_V1;
+        const int fl = KRON_MIXED_FORCE_V0 << v;
         st = build_plan<T>(sub, one, nullptr, fl, stream);
         if (st != KRON_OK) break;
         const int reps = 3;
@@ -274,7 +279,9 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
                               const char *cache, int flags, hipStream_t stream)
 {
     out = nullptr;
-    const int forced = (flags & KRON_MIXED_FORCE_V1) ? 1 : (flags & KRON_MIXED_FORCE_V0) ? 0 : -1;
+    int forced = -1;
+    for (int v = kMixVariants - 1; v >= 0; --v)
+        if (flags & (KRON_MIXED_FORCE_V0 << v)) forced = v;
     const std::string arch = device_arch();
     const int nclass = (int)classes.size();
     auto *p = new MixedPlanCore();
@@ -321,8 +328,9 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
             continue;
         }
         C.variant = v;
-        C.kind = info.big ? KRON_MIXED_KIND_TWOPASS
-                          : info.packed ? KRON_MIXED_KIND_PACKED : KRON_MIXED_KIND_ONCHIP;
+        C.kind = info.slab ? KRON_MIXED_KIND_SLAB
+                 : info.big ? KRON_MIXED_KIND_TWOPASS
+                 : info.packed ? KRON_MIXED_KIND_PACKED : KRON_MIXED_KIND_ONCHIP;
         C.cost = (t.us > 0 && t.variant == v) ? t.us : model_cost_us<T>(cd.d, cd.n, info);
         C.atomic = (info.packed || (flags & KRON_MIXED_ATOMIC)) ? 1 : 0;
         lds = std::max(lds, info.lds);
@@ -435,6 +443,13 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
                 for (int t = 0; t < info.tiles2; ++t)
                     add(MixedItem{c, base + i, j - i, -2 - t},
                         0.6 * C.cost * (j - i) / info.tiles2, 1);
+            } else if (C.kind == KRON_MIXED_KIND_SLAB) {
+                // item t holds slabs t*S .. t*S+S-1 (S = info.slabs); a
+                // group's items are neighbours in the queue (equal costs,
+                // stable sort)
+                const int S = info.slabs;
+                for (int t = 0; t < info.tiles1; ++t)
+                    add(MixedItem{c, base + i, j - i, t * S}, C.cost * (j - i) / info.tiles1, -1);
             } else {
                 add(MixedItem{c, base + i, j - i, -1}, C.cost * (j - i), -1);
             }
@@ -448,7 +463,8 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         ilaunch[w] = l;
         load[l] += icost[w];
     }
-    // each launch's items, largest first
+    // each launch's items, largest first (a group's slabs stay neighbours:
+    // equal costs, stable sort)
     std::vector<MixedItem> sitems;
     int nA = 0;
     for (int l = 0; l < 2; ++l) {
@@ -665,9 +681,12 @@ kron_status mixed_class_info(MixedPlanCore *p, int cls, int *kind, int *variant,
     return KRON_OK;
 }
 
-kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds)
+kron_status mixed_launch_info(MixedPlanCore *p, int *items, int *grid, int *lds, int *launches)
 {
     if (!p) return KRON_EINVAL;
+    if (launches)
+        *launches = (p->L[0].nitems > 0) + (p->L[1].nitems > 0) +
+                    (int)(p->elem == 8 ? p->launch64.size() : p->launch32.size());
     if (items) *items = p->L[0].nitems + p->L[1].nitems;
     if (grid) *grid = std::max(p->L[0].grid, p->L[1].grid);
     if (lds) *lds = (int)p->L[0].lds;

@@ -32,10 +32,10 @@ CLASSES = [
     (5, 6, 20, 8, 7, 7),       # one entry per workgroup (7,776 elements)
     (6, 2, 90, 20, 25, 2),     # packed
     (6, 4, 60, 20, 19, 4),     # one entry per workgroup
-    (6, 5, 10, 5, 4, 5),       # two passes
-    (5, 7, 10, 5, 4, 8),       # two passes
-    (5, 8, 6, 4, 3, 8),        # two passes
-    (6, 6, 6, 3, 3, 6),        # two passes
+    (6, 5, 10, 5, 4, 5),       # slabs (mappings 0, 1) or two passes (mapping 2)
+    (5, 7, 10, 5, 4, 8),       # slabs / two passes
+    (5, 8, 6, 4, 3, 8),        # slabs / two passes
+    (6, 6, 6, 3, 3, 6),        # slabs / two passes
     (3, 9, 12, 6, 5, 9),       # outside the compiled set: separate launch
 ]
 
@@ -126,7 +126,7 @@ def _assert_pools(got, want, exact, tol=1e-12):
 
 
 @pytest.mark.parametrize("flags", [0, K.KRON_MIXED_ATOMIC, K.KRON_MIXED_FORCE_V0,
-                                   K.KRON_MIXED_FORCE_V1])
+                                   K.KRON_MIXED_FORCE_V1, K.KRON_MIXED_FORCE_V2])
 def test_mixed_integer_bit_exact(flags):
     case = _Case(CLASSES, seed=3)
     plan = case.plan(flags=flags)
@@ -134,15 +134,38 @@ def test_mixed_integer_bit_exact(flags):
     _assert_pools(case.got(), case.expected(), exact=True)
     kinds = {(c[0], c[1]): plan.class_info(i)["kind"] for i, c in enumerate(CLASSES)}
     assert kinds[(3, 9)] == "launch"
-    assert kinds[(6, 6)] == "twopass" and kinds[(5, 8)] == "twopass"
+    big = "twopass" if flags == K.KRON_MIXED_FORCE_V2 else "slab"
+    assert kinds[(6, 6)] == big and kinds[(5, 8)] == big
+    # one persistent launch, plus the separate class (two-pass: two launches)
+    assert plan.launch_info()["launches"] == (3 if big == "twopass" else 2)
     assert kinds[(2, 3)] == "packed" and kinds[(1, 5)] == "packed"
-    if flags != K.KRON_MIXED_FORCE_V1:
+    if flags not in (K.KRON_MIXED_FORCE_V1, K.KRON_MIXED_FORCE_V2):
         assert kinds[(6, 4)] == "onchip" and kinds[(5, 6)] == "onchip"
     for i in range(len(CLASSES)):
         info = plan.class_info(i)
         if flags == K.KRON_MIXED_ATOMIC or info["kind"] == "packed":
             assert info["atomic"] == 1
     plan.close()
+
+
+@pytest.mark.parametrize("flags", [K.KRON_MIXED_FORCE_V0, K.KRON_MIXED_FORCE_V1,
+                                   K.KRON_MIXED_FORCE_V0 | K.KRON_MIXED_ATOMIC,
+                                   K.KRON_MIXED_FORCE_V1 | K.KRON_MIXED_ATOMIC])
+def test_mixed_slab_classes_walk(flags):
+    """The four classes too large for LDS as slab items only: as many slabs
+    per item as LDS holds (3, 2, 2 and 1 of 7, 8, 5 and 6: a last item with
+    fewer) or one slab per item (more items than resident workgroups), Y
+    groups of several entries and padded factors; integer-valued, bit-exact."""
+    classes = [(6, 6, 150, 60, 40, 7), (5, 8, 200, 80, 50, 8), (5, 7, 260, 90, 70, 9),
+               (6, 5, 300, 100, 60, 5)]
+    case = _Case(classes, seed=31)
+    plan = case.plan(flags=flags)
+    for i in range(len(classes)):
+        assert plan.class_info(i)["kind"] == "slab"
+    if flags & K.KRON_MIXED_FORCE_V1:  # one slab per item: every workgroup walks items
+        assert plan.launch_info()["items"] > plan.launch_info()["grid"]
+    plan.execute()
+    _assert_pools(case.got(), case.expected(), exact=True)
 
 
 def test_mixed_uniform_f64():
@@ -235,7 +258,7 @@ def test_mixed_autotune_and_cache(tmp_path):
     assert len(lines) == 3  # the separately launched class is not tuned
     for i in range(3):
         info = plan.class_info(i)
-        assert info["variant"] in (0, 1) and info["cost"] > 0
+        assert info["variant"] in (0, 1, 2) and info["cost"] > 0
     plan.execute()
     _assert_pools(case.got(), case.expected(), exact=True)
     # a second plan takes the mappings from the cache without re-timing
@@ -274,6 +297,7 @@ def test_mixed_sparse_grid_workload_shape():
     plan = case.plan()
     info = plan.launch_info()
     assert info["items"] > 0 and info["grid"] > 0
+    assert info["launches"] == 1  # every class in the one persistent launch
     plan.execute()
     _assert_pools(case.got(), case.expected(), exact=False, tol=1e-12)
 
