@@ -29,6 +29,15 @@
  *   - X is never written and W is used only as scratch when a shape does not
  *     fit on chip (the reference clobbers X in its batched layouts; callers
  *     must still treat X/W contents as unspecified after a call there).
+ *   - FP32 (the _f32 entry points and the float overloads) sums each mode's
+ *     n terms in FP32 (FP32 MFMA accumulation for n >= 16), where the
+ *     reference's float templates sum each small GEMM in double and round
+ *     per mode (kgemm_nn.hpp:688-701).  This is a deliberate contract
+ *     change: on the same float inputs the results stay within 8 times the
+ *     reference's own float error against FP64 (1.4x-4.5x measured, 1e-7 to
+ *     2.4e-7 relative in norm; tests/test_gpu_fp32_contract.py), and below it
+ *     where many entries merge into one Y.  The adjoint's gradient sums and
+ *     the kgemm entry points keep FP64 accumulation;
  *   - stream is a hipStream_t (NULL = the null stream).
  *
  * All functions return a kron_status.  Argument errors are detected on the
