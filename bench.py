@@ -605,15 +605,21 @@ def bench_kgemm(args):
     C_b = A_b op(B_b) + C_b (alpha = 1, beta = 1) over contiguous per-entry
     operands, shapes of the recursion's steps at d = 6, n = 4 (mm = 1024,
     nn = kk = 4), a square 16^3 case and 4^3 entries; algorithmic bytes per entry
-    8*(mm*kk + kk*nn + 2*mm*nn)."""
+    8*(mm*kk + kk*nn + 2*mm*nn) (C read and written; once with beta = 0)."""
     import torch
     import kronmult_amd as K
     dev = "cuda:0"
     stream = torch.cuda.current_stream()
     torch.manual_seed(2024)
-    for tr, mm, nn, kk, batch in (("nt", 1024, 4, 4, 65536), ("nn", 1024, 4, 4, 65536),
-                                  ("nn", 16, 16, 16, 262144), ("nt", 16, 16, 16, 262144),
-                                  ("nn", 4, 4, 4, 1 << 20)):
+    # beta = 1 adds alpha*AB into C with device atomics (entries may share C,
+    # kgemm_nn.hpp:392-394); beta = 0 stores it (C is written, not read)
+    for tr, mm, nn, kk, batch, beta in (("nt", 1024, 4, 4, 65536, 1.0),
+                                        ("nn", 1024, 4, 4, 65536, 1.0),
+                                        ("nn", 1024, 4, 4, 65536, 0.0),
+                                        ("nn", 16, 16, 16, 262144, 1.0),
+                                        ("nt", 16, 16, 16, 262144, 1.0),
+                                        ("nn", 16, 16, 16, 262144, 0.0),
+                                        ("nn", 4, 4, 4, 1 << 20, 1.0)):
         A = torch.rand(batch * mm * kk, dtype=torch.float64, device=dev)
         B = torch.rand(batch * kk * nn, dtype=torch.float64, device=dev)
         C = torch.zeros(batch * mm * nn, dtype=torch.float64, device=dev)
@@ -625,11 +631,11 @@ def bench_kgemm(args):
         ldB = torch.full((batch,), kk if tr == "nn" else nn, dtype=torch.int32, device=dev)
         ldC = torch.full((batch,), mm, dtype=torch.int32, device=dev)
         f = K.kgemm_nn_batched if tr == "nn" else K.kgemm_nt_batched
-        ms = _time_steps(lambda: f(mm, nn, kk, 1.0, pA, ldA, pB, ldB, 1.0, pC, ldC, batch,
+        ms = _time_steps(lambda: f(mm, nn, kk, 1.0, pA, ldA, pB, ldB, beta, pC, ldC, batch,
                                    stream=stream), args.steps, args.warmup, stream)
-        alg = 8 * (mm * kk + kk * nn + 2 * mm * nn) * batch
+        alg = 8 * (mm * kk + kk * nn + (2 if beta != 0 else 1) * mm * nn) * batch
         print(json.dumps({"workload": "kgemm_%s_batched" % tr, "mm": mm, "nn": nn, "kk": kk,
-                          "dtype": "f64", "batch": batch, "ms": round(ms, 4),
+                          "beta": beta, "dtype": "f64", "batch": batch, "ms": round(ms, 4),
                           "alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
                           "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}),
               flush=True)
