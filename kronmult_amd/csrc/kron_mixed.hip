@@ -14,15 +14,18 @@
 //     for a class too large for LDS, S slabs of its first mode for a
 //     Y-target group (every slab reads X and adds its own rows of Y, so
 //     items never wait for one another and an apply is one launch);
-//   * the items are sorted by estimated cost, largest first, and taken from
-//     an atomic counter (longest-processing-time-first list scheduling);
+//   * the items are spread over one queue per XCD (the slab items of an
+//     entry on one queue, so their reads of its X share one L2), sorted by
+//     estimated cost, largest first, and taken from atomic counters
+//     (longest-processing-time-first list scheduling); a workgroup drains
+//     its own XCD's queue, then takes from the others;
 //   * each class runs a compile-time (D, N) path with a tile and thread
 //     mapping variant chosen by the autotuner (threads per entry for on-chip
 //     classes, slabs per item for large ones);
 //   * a third mapping of the large classes, two passes through scratch
 //     (slow modes, then fast modes), needs the apply split into two
 //     launches; it is kept for comparison and never chosen by the tuner;
-//   * the counter resets itself (the last workgroup out clears it), so the
+//   * the counters reset themselves (the last workgroup out clears them), so the
 //     launch needs no memset and replays inside a hipGraph.
 //
 // Y update semantics are the reference's (Y += K X, element-wise atomic where
@@ -141,21 +144,55 @@ struct MixCtx {
     const MixedItem *items;
     const MixedEntry *recs;  // per-entry pointers, in item order
     int nitems;
-    unsigned *queue;   // the take counter, then the exit counter
+    unsigned *queue;   // the queues' take counters, then the exit counter
+    MixQueues qs;
     T *scratch;
     T *smem;
     int *s_next;       // LDS slot broadcasting a dequeued index
-    unsigned ahead;    // thread 0: the index requested for the item after next
+    // thread 0: the queue it takes from, how many queues it has found empty,
+    // and the position requested in that queue for the item after next
+    int q, empty;
+    unsigned ahead;
 };
 
-// The next item index (workgroup-uniform).  Thread 0 keeps one request in
-// flight, so the atomic's round trip overlaps the current item.
+// The XCD this wave runs on (hardware register XCC_ID; 0..7 on MI355X).
+__device__ __forceinline__ int mix_xcd()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & (kMixXcds - 1));
+}
+
+// Thread 0: request the next position of queue x.q.
+template <typename T>
+__device__ __forceinline__ void mix_request(MixCtx<T> &x)
+{
+    x.ahead = atomicAdd(x.queue + kMixQueueStride * x.q, 1u);
+}
+
+// The next item index (workgroup-uniform; nitems once every queue is empty).
+// A workgroup takes from its own XCD's queue, then from the following ones
+// in turn.  Thread 0 keeps one request in flight, so the atomic's round trip
+// overlaps the current item; a request past the end of its queue moves it
+// to the next queue.  Every position a request returns below its queue's
+// length is run, so each item runs exactly once.
 template <typename T>
 __device__ __forceinline__ int mix_dequeue(MixCtx<T> &x)
 {
     if (threadIdx.x == 0) {
-        *x.s_next = (int)x.ahead;
-        x.ahead = atomicAdd(x.queue, 1u);
+        int it = x.nitems;
+        while (x.empty < kMixXcds) {
+            const int beg = x.qs.off[x.q], len = x.qs.off[x.q + 1] - beg;
+            if ((int)x.ahead < len) {
+                it = beg + (int)x.ahead;
+                mix_request(x);
+                break;
+            }
+            ++x.empty;
+            x.q = x.q + 1 == kMixXcds ? 0 : x.q + 1;
+            if (x.empty < kMixXcds) mix_request(x);
+        }
+        *x.s_next = it;
     }
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(*x.s_next);
@@ -906,13 +943,16 @@ constexpr int mix_code(int D, int N, int V) { return (D * 16 + N) * kMixVariants
 template <typename T>
 __global__ void __launch_bounds__(kMixThreads, 2)
 kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const MixedEntry *recs,
-                  int nitems, unsigned *queue, T *scratch)
+                  int nitems, unsigned *queue, MixQueues qs, T *scratch)
 {
     extern __shared__ __align__(16) unsigned char kron_smem[];
     __shared__ int s_next;
-    MixCtx<T> x{cls, items, recs, nitems, queue, scratch,
-                reinterpret_cast<T *>(kron_smem), &s_next, 0u};
-    if (threadIdx.x == 0) x.ahead = atomicAdd(queue, 1u);
+    MixCtx<T> x{cls, items, recs, nitems, queue, qs, scratch,
+                reinterpret_cast<T *>(kron_smem), &s_next, 0, 0, 0u};
+    if (threadIdx.x == 0) {
+        x.q = mix_xcd();
+        mix_request(x);
+    }
     int it = mix_dequeue(x);
     while (it < nitems) {
         // lane-dependent index math of every path derives from this opaque
@@ -936,10 +976,10 @@ kron_mixed_kernel(const MixedClass<T> *cls, const MixedItem *items, const MixedE
     // clears the counters for the next launch on the stream
     if (threadIdx.x == 0) {
         __threadfence();
-        const unsigned done = atomicAdd(queue + 1, 1u);
+        const unsigned done = atomicAdd(queue + kMixQueueExit, 1u);
         if (done == gridDim.x - 1) {
-            atomicExch(queue, 0u);
-            atomicExch(queue + 1, 0u);
+            for (int q = 0; q < kMixXcds; ++q) atomicExch(queue + kMixQueueStride * q, 0u);
+            atomicExch(queue + kMixQueueExit, 0u);
         }
     }
 }
@@ -1005,7 +1045,7 @@ kron_status mixed_launch(const MixedLaunch &L, hipStream_t stream)
     }
     hipLaunchKernelGGL(kron_mixed_kernel<T>, dim3((unsigned)L.grid), dim3(kMixThreads), L.lds,
                        stream, static_cast<const MixedClass<T> *>(L.classes), L.items, L.recs,
-                       L.nitems, L.queue, static_cast<T *>(L.scratch));
+                       L.nitems, L.queue, L.q, static_cast<T *>(L.scratch));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_last_hip_error((int)e);

@@ -13,6 +13,21 @@ constexpr int kMixThreads = 256;       // threads per workgroup
 constexpr int kMixLdsBytes = 65536;    // LDS of the largest path (2 workgroups per CU)
 constexpr int kMixVariants = 3;        // tile / thread mapping variants per (d, n)
 
+// Work queues of one launch: one per XCD (MI355X has 8, each with its own
+// L2), so that the items a plan puts on one queue (the slab items of one
+// entry, which all read its X) run on one XCD and share its L2.  Counter q
+// sits at kMixQueueStride * q (one 64-byte line each), the exit counter at
+// kMixQueueExit; kMixQueueWords words per launch, zero between launches.
+constexpr int kMixXcds = 8;
+constexpr int kMixQueueStride = 16;
+constexpr int kMixQueueExit = kMixXcds * kMixQueueStride;
+constexpr int kMixQueueWords = kMixQueueExit + kMixQueueStride;
+
+// Item ranges of the queues: queue q holds items [off[q], off[q + 1]).
+struct MixQueues {
+    int off[kMixXcds + 1];
+};
+
 // One shape class as the kernel reads it (device table).
 template <typename T>
 struct MixedClass {
@@ -64,7 +79,8 @@ struct MixedLaunch {
     const MixedItem *items = nullptr;
     const MixedEntry *recs = nullptr;
     int nitems = 0;
-    unsigned *queue = nullptr;      // two counters, zero between launches
+    unsigned *queue = nullptr;      // kMixQueueWords counters, zero between launches
+    MixQueues q{};                  // the queues' item ranges
     void *scratch = nullptr;        // per-entry scratch of two-pass classes
     int grid = 0;
     size_t lds = 0;

@@ -463,17 +463,54 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         ilaunch[w] = l;
         load[l] += icost[w];
     }
-    // each launch's items, largest first (a group's slabs stay neighbours:
-    // equal costs, stable sort)
+    // each launch's items on one queue per XCD: the slab items of one group
+    // form a bundle that stays on one queue (they all read the group's X, so
+    // on one XCD they share its L2); bundles go to the least loaded queue,
+    // largest first, and each queue runs its items largest first (a bundle's
+    // items stay neighbours: equal costs, stable sort)
     std::vector<MixedItem> sitems;
     int nA = 0;
+    MixQueues qs[2] = {};
     for (int l = 0; l < 2; ++l) {
-        std::vector<int> iord;
-        for (size_t i = 0; i < items.size(); ++i)
-            if (ilaunch[i] == l) iord.push_back((int)i);
-        std::stable_sort(iord.begin(), iord.end(),
-                         [&](int a, int b) { return icost[a] > icost[b]; });
-        for (int i : iord) sitems.push_back(items[i]);
+        std::vector<std::vector<int>> bundles;
+        std::map<std::pair<int, int>, int> slab_bundle;
+        for (size_t i = 0; i < items.size(); ++i) {
+            if (ilaunch[i] != l) continue;
+            const MixedItem &it = items[i];
+            if (p->cls[it.cls].kind == KRON_MIXED_KIND_SLAB) {
+                auto ins = slab_bundle.emplace(std::make_pair(it.cls, it.first),
+                                               (int)bundles.size());
+                if (ins.second) bundles.emplace_back();
+                bundles[ins.first->second].push_back((int)i);
+            } else {
+                bundles.push_back({(int)i});
+            }
+        }
+        std::vector<double> bcost(bundles.size(), 0.0);
+        std::vector<int> bord(bundles.size());
+        for (size_t b = 0; b < bundles.size(); ++b) {
+            bord[b] = (int)b;
+            for (int i : bundles[b]) bcost[b] += icost[i];
+        }
+        std::stable_sort(bord.begin(), bord.end(),
+                         [&](int a, int b) { return bcost[a] > bcost[b]; });
+        std::vector<std::vector<int>> queue(kMixXcds);
+        double qload[kMixXcds] = {};
+        for (int b : bord) {
+            int q = 0;
+            for (int k = 1; k < kMixXcds; ++k)
+                if (qload[k] < qload[q]) q = k;
+            qload[q] += bcost[b];
+            queue[q].insert(queue[q].end(), bundles[b].begin(), bundles[b].end());
+        }
+        const int base = (int)sitems.size();
+        for (int q = 0; q < kMixXcds; ++q) {
+            std::stable_sort(queue[q].begin(), queue[q].end(),
+                             [&](int a, int b) { return icost[a] > icost[b]; });
+            qs[l].off[q] = (int)sitems.size() - base;
+            for (int i : queue[q]) sitems.push_back(items[i]);
+        }
+        qs[l].off[kMixXcds] = (int)sitems.size() - base;
         if (l == 0) nA = (int)sitems.size();
     }
 
@@ -526,9 +563,10 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         }
     }
     put(&dord, recs.data(), sizeof(MixedEntry) * recs.size());
-    if (e == hipSuccess && (e = hipMalloc(&dq, 4 * sizeof(unsigned))) == hipSuccess) {
+    if (e == hipSuccess &&
+        (e = hipMalloc(&dq, 2 * kMixQueueWords * sizeof(unsigned))) == hipSuccess) {
         p->keep(dq);
-        e = hipMemset(dq, 0, 4 * sizeof(unsigned));
+        e = hipMemset(dq, 0, 2 * kMixQueueWords * sizeof(unsigned));
     }
     if (e == hipSuccess && wtotal > 0 && (e = hipMalloc(&dscr, wtotal * sizeof(T))) == hipSuccess)
         p->keep(dscr);
@@ -543,7 +581,8 @@ static kron_status build_plan(MixedPlanCore *&out, const std::vector<ClassDesc> 
         L.items = static_cast<const MixedItem *>(di) + (l == 0 ? 0 : nA);
         L.recs = static_cast<const MixedEntry *>(dord);
         L.nitems = l == 0 ? nA : (int)sitems.size() - nA;
-        L.queue = static_cast<unsigned *>(dq) + 2 * l;
+        L.queue = static_cast<unsigned *>(dq) + kMixQueueWords * l;
+        L.q = qs[l];
         L.lds = lds;
         long long grid = resident > 0 ? resident : 1024;
         grid = std::max(1LL, std::min(grid, (long long)L.nitems));
