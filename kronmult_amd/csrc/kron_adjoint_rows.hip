@@ -185,24 +185,91 @@ kron_adj6_rows_v(A6IO<T, LAY> io, T *__restrict__ W, int C2)
     }
 }
 
+// LDS layouts of a row (slot of element e).  A wave's 64-bit LDS accesses
+// serve 16 lanes at a time over 16 bank pairs; the fibers of one stride S
+// that 16 consecutive threads own must then sit in 16 distinct bank pairs.
+//   Plain: e.  Stride 216 is conflict-free, stride 1 too when a fiber is
+//          read as three 16-byte accesses (8 lanes at a time over 8 bank
+//          quads); strides 6 and 36 put two lanes on one pair.
+//   Pad:   e + 2 (e / 36) (1,366 slots).  Strides 6 and 36 are conflict-free
+//          (fiber f's first slot is f plus a multiple of 16 modulo 16); 1
+//          and 216 are not.
+// Every fiber keeps compile-time offsets between its elements in both.
+enum class A6Lay { Plain, Pad };
+constexpr int kA6Row = kA6C + 2 * (kA6C / 36 - 1);  // slots of a Pad row
+
+template <A6Lay L>
+__device__ __forceinline__ int a6_slot(int e)
+{
+    return L == A6Lay::Pad ? e + 2 * (e / 36) : e;
+}
+// slot distance between consecutive elements of a stride-S fiber
+template <A6Lay L, int S>
+constexpr int a6_step()
+{
+    return L == A6Lay::Plain ? S : S == 1 ? 1 : S == 6 ? 6 : S == 36 ? 38 : 228;
+}
+// 16-byte accesses: stride-1 FP64 fibers of a Plain row (first slot 6 f)
+template <typename T, A6Lay L, int S>
+constexpr bool a6_wide()
+{
+    return L == A6Lay::Plain && S == 1 && sizeof(T) == 8;
+}
+typedef double a6_d2 __attribute__((ext_vector_type(2)));
+
+template <typename T, A6Lay L, int S>
+__device__ __forceinline__ void a6_read_fiber(const T *buf, int base, T (&v)[kA6N])
+{
+    const T *p = buf + a6_slot<L>(base);
+    if constexpr (a6_wide<T, L, S>()) {
+#pragma unroll
+        for (int k = 0; k < kA6N / 2; ++k) {
+            const a6_d2 t = reinterpret_cast<const a6_d2 *>(p)[k];
+            v[2 * k] = t.x;
+            v[2 * k + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kA6N; ++j) v[j] = p[j * a6_step<L, S>()];
+    }
+}
+
+template <typename T, A6Lay L, int S>
+__device__ __forceinline__ void a6_write_fiber(T *buf, int base, const T (&v)[kA6N])
+{
+    T *p = buf + a6_slot<L>(base);
+    if constexpr (a6_wide<T, L, S>()) {
+#pragma unroll
+        for (int k = 0; k < kA6N / 2; ++k) {
+            a6_d2 t;
+            t.x = v[2 * k];
+            t.y = v[2 * k + 1];
+            reinterpret_cast<a6_d2 *>(p)[k] = t;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kA6N; ++j) p[j * a6_step<L, S>()] = v[j];
+    }
+}
+
 // dst fiber f (stride S) = F src fiber f, f < 216 = thread id (in place when
-// dst == src).
-template <typename T, int S>
+// dst == src and the layouts agree); src in layout LI, dst in LO.
+template <typename T, int S, A6Lay LI = A6Lay::Plain, A6Lay LO = A6Lay::Plain>
 __device__ __forceinline__ void a6_contract(const T *src, T *dst, const T (&a)[kA6N][kA6N])
 {
     const int f = threadIdx.x;
     if (f >= kA6F) return;
     const int base = (f / S) * (S * kA6N) + f % S;
-    T in[kA6N];
-#pragma unroll
-    for (int j = 0; j < kA6N; ++j) in[j] = src[base + j * S];
+    T in[kA6N], out[kA6N];
+    a6_read_fiber<T, LI, S>(src, base, in);
 #pragma unroll
     for (int i = 0; i < kA6N; ++i) {
         T s = a[i][0] * in[0];
 #pragma unroll
         for (int j = 1; j < kA6N; ++j) s = __builtin_fma(a[i][j], in[j], s);
-        dst[base + i * S] = T(s);
+        out[i] = s;
     }
+    a6_write_fiber<T, LO, S>(dst, base, out);
 }
 
 // Half of one gradient's positions (fibers of stride S) of the row,
@@ -234,8 +301,9 @@ __device__ __forceinline__ void a6_gram(const T *G, const T *P, int half,
 }
 
 // Rows p0 .. p0+2 of one gradient over every position (fiber of stride S)
-// of the row, lane + 64 i: acc[p][q] += G[p0 + p] P[q].
-template <typename T, int S>
+// of the row, lane + 64 i: acc[p][q] += G[p0 + p] P[q]; G in layout LG, P
+// in LP.
+template <typename T, int S, A6Lay LG = A6Lay::Plain, A6Lay LP = A6Lay::Plain>
 __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
                                              double (&acc)[3][kA6N])
 {
@@ -244,10 +312,30 @@ __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
     for (int f = lane; f < kA6F; f += 64) {
         const int base = (f / S) * (S * kA6N) + f % S;
         double g[3], p[kA6N];
+        const T *gp = G + a6_slot<LG>(base);
+        if constexpr (a6_wide<T, LG, S>()) {
+            // elements p0 .. p0 + 2: one 16-byte pair and a single
+            if (p0 == 0) {
+                const a6_d2 t = reinterpret_cast<const a6_d2 *>(gp)[0];
+                g[0] = t.x;
+                g[1] = t.y;
+                g[2] = gp[2];
+            } else {
+                const a6_d2 t = reinterpret_cast<const a6_d2 *>(gp)[2];
+                g[0] = gp[3];
+                g[1] = t.x;
+                g[2] = t.y;
+            }
+        } else {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) g[i] = double(G[base + (p0 + i) * S]);
+            for (int i = 0; i < 3; ++i) g[i] = double(gp[(p0 + i) * a6_step<LG, S>()]);
+        }
+        {
+            T pv[kA6N];
+            a6_read_fiber<T, LP, S>(P, base, pv);
 #pragma unroll
-        for (int j = 0; j < kA6N; ++j) p[j] = double(P[base + j * S]);
+            for (int j = 0; j < kA6N; ++j) p[j] = double(pv[j]);
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -259,20 +347,22 @@ __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
 // Each gradient is split over a pair of waves by rows (p = 0..2 and 3..5):
 // waves 0, 1 take dA_4 then dA_3, waves 2, 3 dA_5 then dA_6, so that every
 // wave works in both gradient phases and keeps two 3 x 6 blocks of FP64 sums
-// per lane.  Four LDS rows (42 KB: three workgroups per CU):
-//   stage     B0 = P6 (U), B1 = G3 (dY)
-//   round 1   B2 = P5 = A6 . P6 (mode 6),   B3 = G4 = A3^T . G3 (mode 3)
-//   round 2   B0 = P4 = A5 . P5 (mode 5),   B1 = G5 = A4^T . G4 (mode 4)
+// per lane.  Four LDS rows of 1,366 slots (44 KB: three workgroups per CU),
+// each intermediate in the layout its accesses favour (Plain / Pad):
+//   stage     B0 = P6 (U, Plain), B1 = G3 (dY, Plain)
+//   round 1   B2 = P5 = A6 . P6 (mode 6, Pad),   B3 = G4 = A3^T . G3 (mode 3, Pad)
+//   round 2   B0 = P4 = A5 . P5 (mode 5, Pad),   B1 = G5 = A4^T . G4 (mode 4, Pad)
 //   grams     dA_4 (G4, P4), dA_5 (G5, P5)
-//   round 3   B2 = P3 = A4 . P4 (mode 4),   B3 = G6 = A5^T . G5 (mode 5)
+//   round 3   B2 = P3 = A4 . P4 (mode 4, Plain), B3 = G6 = A5^T . G5 (mode 5, Plain)
 //   restage   B0 = U, B1 = dY again, from the registers they were staged from
 //   grams     dA_3 (G3, P3), dA_6 (G6, P6)
 // The next item's rows are loaded into those registers after the restage.
 template <typename T, Layout LAY>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 kron_adj6_rows_grams(A6IO<T, LAY> io, const T *__restrict__ W, int C2, int batch)
 {
-    __shared__ T bufs[4][kA6C];
+    constexpr A6Lay PL = A6Lay::Plain, PD = A6Lay::Pad;
+    __shared__ __align__(16) T bufs[4][kA6Row];
     __shared__ double dAs[4 * 36];
     const int tid = threadIdx.x, wave = tid >> 6, pair = wave >> 1, p0 = (wave & 1) * 3;
     // items (entry * 36 + row) below 2^31: checked by the launcher
@@ -343,31 +433,31 @@ kron_adj6_rows_grams(A6IO<T, LAY> io, const T *__restrict__ W, int C2, int batch
         // One factor (72 SGPRs in FP64) at a time: the scheduling barriers
         // keep the next factor's scalar loads behind the current contraction.
         a6_factor<T, false>(a, io, b, 5);
-        a6_contract<T, 1>(bufs[0], bufs[2], a);  // P5
+        a6_contract<T, 1, PL, PD>(bufs[0], bufs[2], a);  // P5
         __builtin_amdgcn_sched_barrier(0);
         a6_factor<T, true>(a, io, b, 2);
-        a6_contract<T, 216>(bufs[1], bufs[3], a);  // G4
+        a6_contract<T, 216, PL, PD>(bufs[1], bufs[3], a);  // G4
         __syncthreads();
         a6_factor<T, false>(a, io, b, 4);
-        a6_contract<T, 6>(bufs[2], bufs[0], a);  // P4
+        a6_contract<T, 6, PD, PD>(bufs[2], bufs[0], a);  // P4
         __builtin_amdgcn_sched_barrier(0);
         a6_factor<T, true>(a, io, b, 3);
-        a6_contract<T, 36>(bufs[3], bufs[1], a);  // G5
+        a6_contract<T, 36, PD, PD>(bufs[3], bufs[1], a);  // G5
         __syncthreads();
-        if (pair == 0) a6_gram_rows<T, 36>(bufs[3], bufs[0], p0, acc[0]);  // dA_4
-        else a6_gram_rows<T, 6>(bufs[1], bufs[2], p0, acc[0]);            // dA_5
+        if (pair == 0) a6_gram_rows<T, 36, PD, PD>(bufs[3], bufs[0], p0, acc[0]);  // dA_4
+        else a6_gram_rows<T, 6, PD, PD>(bufs[1], bufs[2], p0, acc[0]);            // dA_5
         __syncthreads();
         a6_factor<T, false>(a, io, b, 3);
-        a6_contract<T, 36>(bufs[0], bufs[2], a);  // P3
+        a6_contract<T, 36, PD, PL>(bufs[0], bufs[2], a);  // P3
         __builtin_amdgcn_sched_barrier(0);
         a6_factor<T, true>(a, io, b, 4);
-        a6_contract<T, 6>(bufs[1], bufs[3], a);  // G6
+        a6_contract<T, 6, PD, PL>(bufs[1], bufs[3], a);  // G6
         __syncthreads();
         stage();
         __syncthreads();
         if (it + 1 < r1) load(nb, nr);
-        if (pair == 0) a6_gram_rows<T, 216>(bufs[1], bufs[2], p0, acc[1]);  // dA_3
-        else a6_gram_rows<T, 1>(bufs[3], bufs[0], p0, acc[1]);              // dA_6
+        if (pair == 0) a6_gram_rows<T, 216, PL, PL>(bufs[1], bufs[2], p0, acc[1]);  // dA_3
+        else a6_gram_rows<T, 1, PL, PL>(bufs[3], bufs[0], p0, acc[1]);              // dA_6
         __syncthreads();  // the next item's rows overwrite B0 and B1
         if (it + 1 == r1 || nb != b) flush(b);
         b = nb;
