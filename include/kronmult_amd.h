@@ -13,6 +13,13 @@
  *     (kgemm_nn.hpp:392-394);
  *   - every data pointer and every pointer/leading-dimension array is device
  *     memory owned by the caller; calls never synchronise the stream;
+ *   - targets of atomic adds (Y where entries may share it, C with beta == 1,
+ *     the pointer-layout adjoint's dX and dA) must be coarse-grained device
+ *     memory (hipMalloc, torch): the adds are the hardware's no-return FP64 /
+ *     FP32 atomics, which fine-grained memory (hipHostMalloc, coherent
+ *     managed memory) does not support.  The reference's atomicAdd compiles
+ *     to a compare-and-swap loop that does; this library does not accept
+ *     such memory there;
  *   - pointer-layout calls with batchCount >= 2048 order their entries by Y
  *     target on the device and merge runs before the atomic adds; the sort
  *     scratch (about 16 bytes per entry) is a library-owned device buffer per
