@@ -60,7 +60,7 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane)
     return x;
 }
 
-constexpr int kCountThreads = 1024;
+constexpr int kCountThreads = 512;  // 64 -> 128 workgroups at 65,536 entries: more CUs on the atomics
 constexpr int kCountUnroll = 8;  // used buckets per thread and step of the last workgroup
 
 __global__ void __launch_bounds__(kCountThreads)
