@@ -63,7 +63,7 @@ kgemm_batched_kernel(int mm, int nn, int kk, T alpha, const T *const *Aarray,
         const T *bp = TRANSB ? (B + j) : (B + (long long)j * ldb);
         const long long binc = TRANSB ? ldb : 1;
         for (int k = 0; k < kk; ++k) {
-            c = __builtin_fma(double(ap[(long long)k * lda]), double(bp[k * binc]), c);
+            c = kfma(double(ap[(long long)k * lda]), double(bp[k * binc]), c);
         }
         T *cp = C + i + (long long)j * ldc;
         if constexpr (BETA_MODE == 1) {
@@ -141,12 +141,12 @@ kgemm_tiled_kernel(int mm, int nn, int kk, T alpha, const T *const *Aarray, cons
         for (int q = 0; q < 8; ++q)
 #pragma unroll
             for (int jj = 0; jj < NJ; ++jj)
-                c[jj] = __builtin_fma(double(a[q]), double(bs[(k + q) * NJ + jj]), c[jj]);
+                c[jj] = kfma(double(a[q]), double(bs[(k + q) * NJ + jj]), c[jj]);
     }
     for (; k < kk; ++k) {
         const double a = double(A[k * lda]);
 #pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) c[jj] = __builtin_fma(a, double(bs[k * NJ + jj]), c[jj]);
+        for (int jj = 0; jj < NJ; ++jj) c[jj] = kfma(a, double(bs[k * NJ + jj]), c[jj]);
     }
     T *C = Carray[b] + i + (long long)j0 * ldCarray[b];
     const long long ldc = ldCarray[b];

@@ -266,7 +266,7 @@ __device__ __forceinline__ void a6_contract(const T *src, T *dst, const T (&a)[k
     for (int i = 0; i < kA6N; ++i) {
         T s = a[i][0] * in[0];
 #pragma unroll
-        for (int j = 1; j < kA6N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+        for (int j = 1; j < kA6N; ++j) s = kfma(a[i][j], in[j], s);
         out[i] = s;
     }
     a6_write_fiber<T, LO, S>(dst, base, out);
@@ -295,7 +295,7 @@ __device__ __forceinline__ void a6_gram(const T *G, const T *P, int half,
 #pragma unroll
             for (int i = 0; i < kA6N; ++i)
 #pragma unroll
-                for (int j = 0; j < kA6N; ++j) acc[i][j] = __builtin_fma(g[i], p[j], acc[i][j]);
+                for (int j = 0; j < kA6N; ++j) acc[i][j] = kfma(g[i], p[j], acc[i][j]);
         }
     }
 }
@@ -339,7 +339,7 @@ __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < kA6N; ++j) acc[i][j] = __builtin_fma(g[i], p[j], acc[i][j]);
+            for (int j = 0; j < kA6N; ++j) acc[i][j] = kfma(g[i], p[j], acc[i][j]);
     }
 }
 

@@ -81,7 +81,7 @@ kron_d2_kernel(Entries<T, LAY> ent, D2Shape sh, long long batch)
                 T acc = 0;
 #pragma unroll 4
                 for (int j2 = 0; j2 < sh.n2; ++j2)
-                    acc = __builtin_fma(a2s[k2 + j2 * sh.m2], xs[j1 * sh.n2 + j2], acc);
+                    acc = kfma(a2s[k2 + j2 * sh.m2], xs[j1 * sh.n2 + j2], acc);
                 ts[j1 * sh.m2 + k2] = acc;
             }
         }
@@ -92,7 +92,7 @@ kron_d2_kernel(Entries<T, LAY> ent, D2Shape sh, long long batch)
                 T acc = 0;
 #pragma unroll 4
                 for (int j1 = 0; j1 < sh.n1; ++j1)
-                    acc = __builtin_fma(a1s[k1 + j1 * sh.m1], ts[j1 * sh.m2 + k2], acc);
+                    acc = kfma(a1s[k1 + j1 * sh.m1], ts[j1 * sh.m2 + k2], acc);
                 return acc;
             };
             int k1 = p;

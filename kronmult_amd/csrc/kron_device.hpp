@@ -16,6 +16,15 @@
 
 namespace kron {
 
+// Fused multiply-add in the operands' own type.  __builtin_fma is the double
+// builtin: float operands would be widened, summed in double and narrowed
+// again, three instructions per term instead of one.
+__device__ __forceinline__ float kfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double kfma(double a, double b, double c)
+{
+    return __builtin_fma(a, b, c);
+}
+
 constexpr int cpow(int n, int p)
 {
     int r = 1;
@@ -261,7 +270,7 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
 #pragma unroll
                 for (int j = 0; j < NMAX; ++j)
                     if (j < n)
-                        acc = __builtin_fma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
+                        acc = kfma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
                                             in[j], acc);
                 dd[i * inner] = T(acc);
             }
@@ -277,7 +286,7 @@ __device__ __forceinline__ void contract_fibers(const T *__restrict__ src, T *__
             const T *s = src + fiber_base(o, ii, n * inner);
             Acc acc = 0;
             for (int j = 0; j < n; ++j)
-                acc = __builtin_fma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
+                acc = kfma(TRANS ? Acc(Al[j + i * ldA]) : Acc(Al[i + j * ldA]),
                                     Acc(s[j * inner]), acc);
             dst[w] = T(acc);
         }
@@ -310,7 +319,7 @@ __device__ __forceinline__ void contract_fibers_reg(const T *__restrict__ src,
         for (int i = 0; i < N; ++i) {
             Acc acc = a[i][0] * in[0];
 #pragma unroll
-            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
+            for (int j = 1; j < N; ++j) acc = kfma(a[i][j], in[j], acc);
             dd[i * inner] = T(acc);
         }
     }
@@ -356,7 +365,7 @@ __device__ __forceinline__ void contract_rowblocks(const T *__restrict__ src,
         for (int r = 0; r < R; ++r) {
             Acc acc = a[r][0] * in[0];
 #pragma unroll
-            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[r][j], in[j], acc);
+            for (int j = 1; j < N; ++j) acc = kfma(a[r][j], in[j], acc);
             if (i0 + r < N) dd[(i0 + r) * inner] = T(acc);
         }
     }

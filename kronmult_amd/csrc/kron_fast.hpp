@@ -97,7 +97,7 @@ __device__ __forceinline__ void tile_pass(int g, const Acc (&a)[N][N], LoadF &&l
             for (int i = 0; i < N; ++i) {
                 Acc s = a[i][0] * in[0];
 #pragma unroll
-                for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+                for (int j = 1; j < N; ++j) s = kfma(a[i][j], in[j], s);
                 st(base + i * S, s);
             }
         }
@@ -147,7 +147,7 @@ __device__ __forceinline__ void tile_pass_in(int g, const Acc (&a)[N][N], LoadF 
                     Acc s = a[i][0] * Acc(in[bi & 1][q][0]);
 #pragma unroll
                     for (int j = 1; j < N; ++j)
-                        s = __builtin_fma(a[i][j], Acc(in[bi & 1][q][j]), s);
+                        s = kfma(a[i][j], Acc(in[bi & 1][q][j]), s);
                     st(base + i * S, s);
                 }
             }
@@ -206,7 +206,7 @@ __device__ __forceinline__ void tile_pass_y(int g, const Acc (&a)[N][N], LoadF &
                 for (int i = 0; i < N; ++i) {
                     Acc s = a[i][0] * in[0];
 #pragma unroll
-                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+                    for (int j = 1; j < N; ++j) s = kfma(a[i][j], in[j], s);
                     const int e = base + i * S;
                     if (ok(e)) y[yi(e)] = T(Acc(yv[bi & 1][q][i]) + s);
                 }
@@ -395,7 +395,7 @@ kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
                                 Acc s = a[i][0] * Acc(nxt.v[q][0]);
 #pragma unroll
                                 for (int j = 1; j < N; ++j)
-                                    s = __builtin_fma(a[i][j], Acc(nxt.v[q][j]), s);
+                                    s = kfma(a[i][j], Acc(nxt.v[q][j]), s);
                                 L(base + i * S2) = T(s);
                             }
                         }
@@ -437,7 +437,7 @@ kron_fast_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
                         for (int i = 0; i < N; ++i) {
                             Acc s = a[i][0] * in[0];
 #pragma unroll
-                            for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+                            for (int j = 1; j < N; ++j) s = kfma(a[i][j], in[j], s);
                             acc[q][i] += s;
                         }
                     }

@@ -36,7 +36,7 @@ kron_gemv_kernel(Entries<T, LAY> ent, int m, int n, int ld, int seg, long long b
         for (int i = i0; i < m; i += seg) {
             T acc = 0;
 #pragma unroll 8
-            for (int j = 0; j < n; ++j) acc = __builtin_fma(A[i + (long long)j * ld], x[j], acc);
+            for (int j = 0; j < n; ++j) acc = kfma(A[i + (long long)j * ld], x[j], acc);
             y_accumulate<SHARED, T, T>(y, i, acc);
         }
     }

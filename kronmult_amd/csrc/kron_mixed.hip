@@ -372,7 +372,7 @@ struct PathOnchip {
                 for (int i = 0; i < N; ++i) {
                     Acc s = a[i][0] * Acc(rg.v[it][0]);
 #pragma unroll
-                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                    for (int j = 1; j < N; ++j) s = kfma(a[i][j], Acc(rg.v[it][j]), s);
                     L(x, base + i * S2) = T(s);
                 }
             }
@@ -474,7 +474,7 @@ struct PathPacked {
                         Acc s = a[i][0] * Acc(rg.v[it][0]);
 #pragma unroll
                         for (int j = 1; j < N; ++j)
-                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                            s = kfma(a[i][j], Acc(rg.v[it][j]), s);
                         if constexpr (D == 1)
                             atomic_add(y + (base + i * FS), T(s));
                         else
@@ -597,7 +597,7 @@ struct PathPass1 {
                 for (int i = 0; i < N; ++i) {
                     Acc s = a[i][0] * Acc(rg.v[it][0]);
 #pragma unroll
-                    for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                    for (int j = 1; j < N; ++j) s = kfma(a[i][j], Acc(rg.v[it][j]), s);
                     const int t = base + i * C1;
                     if constexpr (H == 1) {
                         if (valid(t, c0)) W[gidx(t, c0)] = T(s);
@@ -728,7 +728,7 @@ struct PathPass2 {
                         Acc s = a[i][0] * Acc(rg.v[it][0]);
 #pragma unroll
                         for (int j = 1; j < N; ++j)
-                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                            s = kfma(a[i][j], Acc(rg.v[it][j]), s);
                         const int t = f * N + i;
                         if (t < nv) {
                             if (atomic)
@@ -749,7 +749,7 @@ struct PathPass2 {
                         Acc s = a[i][0] * Acc(rg.v[it][0]);
 #pragma unroll
                         for (int j = 1; j < N; ++j)
-                            s = __builtin_fma(a[i][j], Acc(rg.v[it][j]), s);
+                            s = kfma(a[i][j], Acc(rg.v[it][j]), s);
                         x.smem[f * N + i] = T(s);
                     }
                 }
@@ -876,7 +876,7 @@ struct PathSlab {
                         Acc v = a1[sl][0] * Acc(in[bi & 1][q][0]);
 #pragma unroll
                         for (int j = 1; j < N; ++j)
-                            v = __builtin_fma(a1[sl][j], Acc(in[bi & 1][q][j]), v);
+                            v = kfma(a1[sl][j], Acc(in[bi & 1][q][j]), v);
                         L(x, sl * F + e) = T(v);
                     }
                 }

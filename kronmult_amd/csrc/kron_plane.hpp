@@ -42,7 +42,7 @@ __device__ __forceinline__ void plane_first(Acc (&v)[4][4], const Acc (&a)[4][4]
         for (int i = 0; i < 4; ++i) {
             Acc s = a[i][0] * v[0][j];
 #pragma unroll
-            for (int k = 1; k < 4; ++k) s = __builtin_fma(a[i][k], v[k][j], s);
+            for (int k = 1; k < 4; ++k) s = kfma(a[i][k], v[k][j], s);
             t[i] = s;
         }
 #pragma unroll
@@ -61,7 +61,7 @@ __device__ __forceinline__ void plane_second(Acc (&v)[4][4], const Acc (&a)[4][4
         for (int j = 0; j < 4; ++j) {
             Acc s = a[j][0] * v[i][0];
 #pragma unroll
-            for (int k = 1; k < 4; ++k) s = __builtin_fma(a[j][k], v[i][k], s);
+            for (int k = 1; k < 4; ++k) s = kfma(a[j][k], v[i][k], s);
             t[j] = s;
         }
 #pragma unroll
@@ -291,9 +291,9 @@ kron_plane_kernel(Entries<T, LAY> ent, int lda, int batch, GroupOrder ord)
                 Acc t[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    Acc s = __builtin_fma(a[q][0], v[i][0], acc[i][q]);
+                    Acc s = kfma(a[q][0], v[i][0], acc[i][q]);
 #pragma unroll
-                    for (int k = 1; k < 4; ++k) s = __builtin_fma(a[q][k], v[i][k], s);
+                    for (int k = 1; k < 4; ++k) s = kfma(a[q][k], v[i][k], s);
                     t[q] = s;
                 }
 #pragma unroll

@@ -76,7 +76,7 @@ kron_small_kernel(Entries<T, Layout::Ptr> ent, SmallShape sh, long long batch)
                     const T *s = src + fiber_base(o, ii, nq * inner);
                     T acc = 0;
                     for (int j = 0; j < nq; ++j)
-                        acc = __builtin_fma(Aq[i + j * mq], s[j * inner], acc);
+                        acc = kfma(Aq[i + j * mq], s[j * inner], acc);
                     if (q + 1 < d)
                         dst[w] = acc;
                     else

@@ -60,7 +60,7 @@ __device__ __forceinline__ void split_contract_regs(Acc (&v)[LEN], const Acc (&a
         for (int i = 0; i < N; ++i) {
             Acc s = a[i][0] * v[base];
 #pragma unroll
-            for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], v[base + j * S], s);
+            for (int j = 1; j < N; ++j) s = kfma(a[i][j], v[base + j * S], s);
             out[i] = s;
         }
 #pragma unroll
@@ -83,7 +83,7 @@ __device__ __forceinline__ void split_contract_lds(T *row, const Acc (&a)[N][N])
         for (int i = 0; i < N; ++i) {
             Acc s = a[i][0] * in[0];
 #pragma unroll
-            for (int j = 1; j < N; ++j) s = __builtin_fma(a[i][j], in[j], s);
+            for (int j = 1; j < N; ++j) s = kfma(a[i][j], in[j], s);
             row[base + i * S] = T(s);
         }
     }
