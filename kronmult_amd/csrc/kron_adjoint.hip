@@ -71,7 +71,7 @@ __device__ __forceinline__ void adj_gram(const T *__restrict__ G, const T *__res
             double s = 0;
             for (int k = sub; k < rest; k += per) {
                 const int base = (o * n) * inner + ii;
-                s = __builtin_fma(double(G[base + p * inner]), double(P[base + r * inner]), s);
+                s = kfma(double(G[base + p * inner]), double(P[base + r * inner]), s);
                 o += d_o;
                 ii += d_ii;
                 if (ii >= inner) {
@@ -107,7 +107,7 @@ __device__ __forceinline__ void adj_gram_row(const T *__restrict__ G, const T *_
         const T *Pb = P + base;
         const double g = double(Gp[base]);
 #pragma unroll
-        for (int r = 0; r < N; ++r) acc[r] = __builtin_fma(g, double(Pb[r * inner]), acc[r]);
+        for (int r = 0; r < N; ++r) acc[r] = kfma(g, double(Pb[r * inner]), acc[r]);
     }
 }
 
@@ -141,7 +141,7 @@ __device__ __forceinline__ void adj_contract(const T *__restrict__ src, T *__res
         for (int i = 0; i < N; ++i) {
             T acc = a[i][0] * in[0];
 #pragma unroll
-            for (int j = 1; j < N; ++j) acc = __builtin_fma(a[i][j], in[j], acc);
+            for (int j = 1; j < N; ++j) acc = kfma(a[i][j], in[j], acc);
             dd[i * inner] = acc;
         }
     }
@@ -651,9 +651,32 @@ static bool adjoint_shape(int d, int n, int elem, AdjShape &sh)
 template <typename T>
 size_t adjoint_scratch_elems(int d, int n)
 {
+    if (d == 6 && n == 4) return 0;  // on chip (kron_adjoint_plane.hip)
     AdjShape sh;
     if (!adjoint_shape(d, n, (int)sizeof(T), sh)) return (size_t)-1;
     return (sh.h < d) ? (size_t)(2 * sh.E) : 0;
+}
+
+// Entries of a pointer-layout d = 6, n = 4 adjoint ordered by dX target
+// (pdX), so that runs adding into one X gradient sum it in registers; from
+// 2,048 entries, as the forward groups by Y.  False: per-entry atomics.
+template <typename T>
+static bool adjoint_plane_order(const AdjointArgs<T> &a, GroupOrder &ord)
+{
+    if (a.layout != Layout::Ptr || a.batch < 2048) return false;
+    const size_t bytes = group_workspace_bytes(a.batch);
+    bool fresh = false;
+    void *ws = stream_workspace(a.stream, bytes, &fresh);
+    if (!ws) return false;
+    const unsigned long long *keys = nullptr;
+    const int *perm = nullptr;
+    if (group_by_target(reinterpret_cast<const void *const *>(a.pdX), a.batch, ws, bytes, fresh,
+                        &keys, &perm, a.stream) != KRON_OK)
+        return false;
+    ord.perm = perm;
+    ord.keys = keys;
+    ord.chunk = 0;
+    return true;
 }
 
 template <typename T, int N, int P, Layout LAY>
@@ -693,7 +716,8 @@ static kron_status launch_adjoint_io(const AdjointArgs<T> &a, const AdjIO<T, LAY
     T *W = a.W;
     AdjShape sh;
     if (!adjoint_shape(d, n, (int)sizeof(T), sh)) return KRON_ENOTSUP;
-    if (sh.h < d && !W) return KRON_EWORKSPACE;
+    const bool plane = d == 6 && n == 4;  // on chip, no scratch
+    if (!plane && sh.h < d && !W) return KRON_EWORKSPACE;
     const int nf = d - sh.h;
     if constexpr (!AdjIO<T, LAY>::kAccumulate) {
         // dA is accumulated atomically by the work items: clear it first (a
@@ -702,6 +726,12 @@ static kron_status launch_adjoint_io(const AdjointArgs<T> &a, const AdjIO<T, LAY
         hipLaunchKernelGGL(kron_fill_zero<T>,
                            dim3((unsigned)std::min<long long>((nA + 255) / 256, 4096)), dim3(256),
                            0, a.stream, a.dA, nA);
+    }
+    // d = 6, n = 4: every entry on chip, one launch (kron_adjoint_plane.hip)
+    if (plane) {
+        GroupOrder ord{nullptr, nullptr, 0};
+        const bool grouped = adjoint_plane_order<T>(a, ord);
+        return launch_adjoint_plane<T>(a, grouped ? &ord : nullptr);
     }
     const long long fixed = adj_fixed_bytes(d, n, (int)sizeof(T));
     // d = 6, n = 6: compile-time kernels for the three passes

@@ -183,6 +183,11 @@ size_t adjoint_scratch_elems(int d, int n);
 // KRON_ENOTSUP when batchCount * 36 exceeds the int range.
 template <typename T>
 kron_status launch_adjoint_rows6(int pass, const AdjointArgs<T> &a);
+// The d = 6, n = 4 adjoint on chip (kron_adjoint_plane.hip); ord: the
+// pointer layout's entries in dX-target order (nullptr: per-entry order).
+// dA must be cleared beforehand in the batched layout.
+template <typename T>
+kron_status launch_adjoint_plane(const AdjointArgs<T> &a, const GroupOrder *ord);
 
 // The reference's per-entry vbatched structure (kron_plain.hip): one
 // workgroup per entry, every mode a pass through the entry's scratch slot of

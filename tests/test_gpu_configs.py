@@ -234,7 +234,8 @@ def _gpu_adjoint(d, n, A, X, dY):
     return host(dXd), host(dAd)
 
 
-@pytest.mark.parametrize("d,n,batch", [(6, 6, 211), (4, 4, 9001), (5, 7, 150), (3, 5, 20011)])
+@pytest.mark.parametrize("d,n,batch", [(6, 6, 211), (4, 4, 9001), (5, 7, 150), (3, 5, 20011),
+                                       (6, 4, 3331)])
 def test_adjoint_walk_integer(d, n, batch):
     """Many items per workgroup (entries x chunks above twice the resident
     grid): integer inputs, so dX and dA are exact whatever the order."""

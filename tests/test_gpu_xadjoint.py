@@ -99,6 +99,8 @@ INT_CASES = [
     (4, 4, 4099, 31, 300, 50),
     (5, 3, 777, 15, 60, 9),
     (6, 4, 513, 40, 70, 20),
+    (6, 4, 4099, 60, 300, 100),  # d6n4 on chip, entries in dX-target order
+    (6, 4, 2048, 2048, 7, 2048),  # a few X vectors shared by ~290 entries each
     (4, 7, 120, 25, 30, 8),
     (5, 5, 97, 17, 21, 7),
     (3, 10, 64, 19, 23, 9),
@@ -143,7 +145,7 @@ def test_xadjoint_matches_batched_adjoint(d, n, batch):
     assert H.rel_err(host(dAp).reshape(batch, d, -1), host(dAb)) < 1e-12
 
 
-@pytest.mark.parametrize("d,n,batch", [(6, 4, 300), (6, 6, 40), (3, 8, 500)])
+@pytest.mark.parametrize("d,n,batch", [(6, 4, 300), (6, 4, 2500), (6, 6, 40), (3, 8, 500)])
 def test_xadjoint_f32(d, n, batch):
     c32 = _pool_case(d, n, batch, 23, 29, 7, n, n, 50 + n, integer=False, dtype=np.float32)
     c64 = dict(c32)
