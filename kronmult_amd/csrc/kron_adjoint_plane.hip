@@ -10,19 +10,22 @@
 // G[.. i_k = p ..] P[.. j_k = q ..], where every other factor is applied on
 // one side (transposed to dY, or to X).  Each pair of modes takes its
 // gradients in the orientation whose planes already hold both sides:
-//   modes 5, 6 (B planes): D1 with X1 contracted by the partner mode;
-//   modes 3, 1 (C planes): X2 with dY contracted by the partner mode;
-//   modes 2, 4 (A planes): D2 with X0 contracted by the partner mode.
+//   modes 5, 6 (B planes): D1 and X1;
+//   modes 3, 1 (C planes): dY and X2;
+//   modes 2, 4 (A planes): D2 and X0;
+// with the pair's other mode (the partner) contracted into one side.
 // A thread's plane gives a 4 x 4 partial sum per gradient (64 FMAs); the
 // workgroup's 256 partials are summed by a wave reduce-scatter (each lane
 // keeps half of its values and adds its partner's half: 8 and 4 pairs by lane
-// swaps across 32 and 16 lanes, 2 and 1 by shuffles, then two full steps
-// within quads) and one LDS add per output and wave.
+// swaps across 32 and 16 lanes, 2 and 1 by DPP lane moves, then two full
+// steps within quads) and one LDS add per output and wave.
 //
 // Per entry: X0 (A planes) and dY (C planes) come from HBM into registers,
 // X1 and D1 go to two LDS buffers, are read back as B planes, X2 and D2 are
 // written in place and read as C and A planes, beside dY and X0 read a second
 // time (from cache); dX leaves from the A planes.
+// Of the six partner contractions, two are a chain's own next step (A6^T D1
+// and A4^T D2).
 // In the pointer layout the entries are taken in the order of their dX
 // targets (as the forward orders by Y), so a run of entries adding into one
 // X gradient sums it in registers and flushes once; dA goes out per entry
@@ -115,23 +118,25 @@ __device__ __forceinline__ void ap_swap_add(double &a, double b)
     }
 }
 
-// a <- a + b by lane pairs L, L ^ M: lanes with bit M clear keep the sum of
-// a, the others the sum of b.
-template <int M>
-__device__ __forceinline__ void ap_pair_add(double &a, double b, bool up)
-{
-    const double mine = up ? b : a, other = up ? a : b;
-    a = mine + __shfl_xor(other, M, 64);
-}
-
-// x + (x of lane L ^ 2, or L ^ 1) within each quad, by DPP
+// x of the partner lane under DPP control CTRL (a permutation within each
+// row of 16 lanes)
 template <int CTRL>
-__device__ __forceinline__ double ap_quad_add(double x)
+__device__ __forceinline__ double ap_dpp(double x)
 {
     unsigned lo, hi;
     ap_split(x, lo, hi);
-    return x + ap_join((unsigned)__builtin_amdgcn_mov_dpp((int)lo, CTRL, 0xF, 0xF, true),
-                       (unsigned)__builtin_amdgcn_mov_dpp((int)hi, CTRL, 0xF, 0xF, true));
+    return ap_join((unsigned)__builtin_amdgcn_mov_dpp((int)lo, CTRL, 0xF, 0xF, true),
+                   (unsigned)__builtin_amdgcn_mov_dpp((int)hi, CTRL, 0xF, 0xF, true));
+}
+
+// a <- a + b by lane pairs (DPP CTRL pairs every lane with one whose bit
+// `up` differs, inside its group of 8 or 16): lanes with the bit clear keep
+// the sum of a, the others the sum of b.
+template <int CTRL>
+__device__ __forceinline__ void ap_pair_add(double &a, double b, bool up)
+{
+    const double mine = up ? b : a, other = up ? a : b;
+    a = mine + ap_dpp<CTRL>(other);
 }
 
 // Sum of v[0..15] over the wave's 64 lanes, reduce-scatter: afterwards every
@@ -152,10 +157,14 @@ __device__ __forceinline__ double ap_wave_reduce16(double (&v)[16], int lane)
     ap_swap_add<16>(v[1], v[5]);
     ap_swap_add<16>(v[2], v[6]);
     ap_swap_add<16>(v[3], v[7]);
-    ap_pair_add<8>(v[0], v[2], (lane & 8) != 0);
-    ap_pair_add<8>(v[1], v[3], (lane & 8) != 0);
-    ap_pair_add<4>(v[0], v[1], (lane & 4) != 0);
-    return ap_quad_add<0xB1>(ap_quad_add<0x4E>(v[0]));  // quad_perm [2,3,0,1], [1,0,3,2]
+    // row_ror:8 pairs lanes L, L ^ 8; row_half_mirror pairs L, L ^ 7 inside
+    // each 8 lanes (bit 2 differs), and the quad steps below then cover the
+    // partner's quad
+    ap_pair_add<0x128>(v[0], v[2], (lane & 8) != 0);
+    ap_pair_add<0x128>(v[1], v[3], (lane & 8) != 0);
+    ap_pair_add<0x141>(v[0], v[1], (lane & 4) != 0);
+    const double r = v[0] + ap_dpp<0x4E>(v[0]);  // quad_perm [2,3,0,1]: L ^ 2
+    return r + ap_dpp<0xB1>(r);                  // quad_perm [1,0,3,2]: L ^ 1
 }
 
 // The workgroup's gradient of factor kf (outputs p + 4 q) added into LDS:
@@ -306,22 +315,22 @@ kron_adjoint_plane_kernel(AdjIO<T, LAY> io, int batch, GroupOrder ord)
                 }
             load_factor<T, Acc, 4, false, true>(a, fac(b, 5), lda);  // A6
             load_factor<T, Acc, 4, false, true>(c, fac(b, 4), lda);  // A5
-            // dA5: D1 with A6 X1 over mode 5 (the second index)
+            // Each gradient takes its partner mode from a chain's own first
+            // step: dA6 (mode 6, the first index) D1 with A5 X1; dA5 (mode 5,
+            // the second index) A6^T D1 with X1.
             ap_copy(p, v);
+            ap_second<false>(p, c);  // A5 X1
+            ap_gram_first(g, p, dAs, 5, l);
+            ap_first<true>(g, a);    // A6^T D1
+            ap_gram_second(g, v, dAs, 4, l);
+            // X2 = A6 (A5 X1), D2 = A5^T (A6^T D1)
             ap_first<false>(p, a);
-            ap_gram_second(g, p, dAs, 4, l);
-            // dA6: D1 with A5 X1 over mode 6 (the first index); X2 = A6 (A5 X1)
-            ap_second<false>(v, c);
-            ap_gram_first(g, v, dAs, 5, l);
-            ap_first<false>(v, a);
-            // D2 = A5^T A6^T D1
-            ap_first<true>(g, a);
             ap_second<true>(g, c);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    LXB[i + 4 * j] = T(v[i][j]);
+                    LXB[i + 4 * j] = T(p[i][j]);
                     LDB[i + 4 * j] = T(g[i][j]);
                 }
         }
@@ -352,8 +361,8 @@ kron_adjoint_plane_kernel(AdjIO<T, LAY> io, int batch, GroupOrder ord)
             ap_first<true>(y0, a);
             ap_gram_second(y0, v, dAs, 0, l);
         }
-        // A planes (first m2 = factor 1, second m4 = factor 3): D2 with X0
-        // contracted by the partner mode, and dX = A2^T A4^T D2
+        // A planes (first m2 = factor 1, second m4 = factor 3): D2 and X0,
+        // and dX = A2^T A4^T D2
         {
             const T *const LDA = ld + plane_pad(eA);
             const auto x = io.x(b) + eA;
@@ -367,13 +376,14 @@ kron_adjoint_plane_kernel(AdjIO<T, LAY> io, int batch, GroupOrder ord)
                 }
             load_factor<T, Acc, 4, false, true>(a, fac(b, 1), lda);  // A2
             load_factor<T, Acc, 4, false, true>(c, fac(b, 3), lda);  // A4
+            // dA4 (mode 4, the second index): D2 with A2 X0; dA2 (mode 2, the
+            // first index): A4^T D2, the first step of dX, with X0
             ap_copy(v, x0);
-            ap_second<false>(v, c);
-            ap_gram_first(g, v, dAs, 1, l);  // dA2 over mode 2
-            ap_first<false>(x0, a);
-            ap_gram_second(g, x0, dAs, 3, l);  // dA4 over mode 4
-            ap_first<true>(g, a);
-            ap_second<true>(g, c);
+            ap_first<false>(v, a);  // A2 X0
+            ap_gram_second(g, v, dAs, 3, l);
+            ap_second<true>(g, c);  // A4^T D2
+            ap_gram_first(g, x0, dAs, 1, l);
+            ap_first<true>(g, a);   // dX = A2^T (A4^T D2)
             if constexpr (GROUPED) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
