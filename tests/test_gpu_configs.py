@@ -246,7 +246,7 @@ def test_adjoint_walk_integer(d, n, batch):
     assert np.array_equal(dA.reshape(rdA.shape), rdA)
 
 
-@pytest.mark.parametrize("d,n,batch", [(6, 6, 96), (4, 4, 4099)])
+@pytest.mark.parametrize("d,n,batch", [(6, 6, 96), (4, 4, 4099), (6, 4, 2100)])
 def test_adjoint_walk_f32(d, n, batch):
     """FP32 adjoint with many items per workgroup, against the FP64 formula on
     the same FP32 inputs."""
