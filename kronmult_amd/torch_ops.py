@@ -44,8 +44,14 @@ class KronMatvec(torch.autograd.Function):
         Ac = _colmajor(A)
         Xc = X.contiguous()
         Y = torch.zeros_like(Xc)
-        W = torch.empty_like(Xc)
-        kronmult_batched(d, n, Ac, Xc, Y, W, batch)
+        try:
+            kronmult_batched(d, n, Ac, Xc, Y, None, batch)
+        except KronError as e:
+            if e.status != KRON_EWORKSPACE:
+                raise
+            # entries too large for the chip stream through W (the call above
+            # was refused before any launch)
+            kronmult_batched(d, n, Ac, Xc, Y, torch.empty_like(Xc), batch)
         ctx.save_for_backward(Ac, Xc)
         ctx.dims = (batch, d, n)
         return Y
