@@ -120,7 +120,7 @@ def test_xadjoint_integer_bit_exact(d, n, batch, na, nx, ny):
     assert np.array_equal(dA, wdA)
 
 
-@pytest.mark.parametrize("d,n,batch", [(6, 6, 211), (4, 4, 2049), (3, 5, 3331)])
+@pytest.mark.parametrize("d,n,batch", [(6, 6, 211), (4, 4, 2049), (3, 5, 3331), (6, 4, 3001)])
 def test_xadjoint_matches_batched_adjoint(d, n, batch):
     """Private factors, X vectors and Y targets (the maps are the identity):
     the pointer layout adds what the batched layout overwrites (both sum the
@@ -257,6 +257,25 @@ def test_kron_xmatvec_large_entries():
     c["dA0"] = np.zeros_like(c["dA0"])
     wdX, wdA = _expected(c)
     # the wrapper takes factors as (row, column) matrices
+    Am = torch.from_numpy(c["Ap"].reshape(na, n, n)).transpose(-1, -2).contiguous()
+    Ag = Am.to(DEV).requires_grad_(True)
+    Xg = torch.from_numpy(c["Xp"]).to(DEV).requires_grad_(True)
+    tmap = lambda m: torch.from_numpy(m).to(DEV)  # noqa: E731
+    Y = kron_xmatvec(Ag, tmap(c["amap"]), Xg, tmap(c["xmap"]), tmap(c["ymap"]), ny)
+    (Y * torch.from_numpy(c["dYp"]).to(DEV)).sum().backward()
+    assert np.array_equal(host(Xg.grad), wdX)
+    assert np.array_equal(host(Ag.grad.transpose(-1, -2).contiguous()).reshape(na, -1), wdA)
+
+
+def test_kron_xmatvec_d6n4_pooled():
+    """The headline shape through the wrapper, 2,500 entries over shared pools
+    (the adjoint runs on chip, in dX-target order): integer data, bit-exact
+    against the per-mode formula."""
+    d, n, batch, na, nx, ny = 6, 4, 2500, 400, 300, 64
+    c = _pool_case(d, n, batch, na, nx, ny, n, n, 646)
+    c["dX0"] = np.zeros_like(c["dX0"])
+    c["dA0"] = np.zeros_like(c["dA0"])
+    wdX, wdA = _expected(c)
     Am = torch.from_numpy(c["Ap"].reshape(na, n, n)).transpose(-1, -2).contiguous()
     Ag = Am.to(DEV).requires_grad_(True)
     Xg = torch.from_numpy(c["Xp"]).to(DEV).requires_grad_(True)
