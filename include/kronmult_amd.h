@@ -21,13 +21,16 @@
  *     to a compare-and-swap loop that does; this library does not accept
  *     such memory there;
  *   - pointer-layout calls with batchCount >= 2048 order their entries by Y
- *     target on the device and merge runs before the atomic adds; the sort
- *     scratch (about 16 bytes per entry) is a library-owned device buffer per
- *     (device, stream), allocated on first use and kept for the process;
+ *     target on the device and merge runs before the atomic adds (the
+ *     pointer-layout adjoint at d = 6, n = 4 orders them by dX target); the
+ *     sort scratch (about 16 bytes per entry) is a library-owned device
+ *     buffer per (device, stream), allocated on first use and kept for the
+ *     process;
  *   - which calls allocate device memory (hipMalloc, never hipFree, never a
- *     synchronisation): kron_xbatched_* / kron_pbatched_* / kron_vbatched_*
- *     when the stream's sort workspace must grow (the first grouped call on
- *     a stream, or a larger batch than before), kron_pbatched_* when its
+ *     synchronisation): kron_xbatched_* (the adjoint included) /
+ *     kron_pbatched_* / kron_vbatched_* when the stream's sort workspace
+ *     must grow (the first grouped call on a stream, or a larger batch than
+ *     before), kron_pbatched_* when its
  *     factor-pointer table must grow, and the plan create calls.  A call
  *     that allocates must not be made under stream capture; warm the stream
  *     up with an uncaptured call of the largest batch first, or use

@@ -104,6 +104,7 @@ __device__ __forceinline__ double ap_join(unsigned lo, unsigned hi)
 template <int M>
 __device__ __forceinline__ void ap_swap_add(double &a, double b)
 {
+    static_assert(M == 32 || M == 16, "lane swaps exist across 32 and 16 lanes");
     unsigned al, ah, bl, bh;
     ap_split(a, al, ah);
     ap_split(b, bl, bh);
