@@ -350,10 +350,10 @@ __device__ __forceinline__ void a6_gram_rows(const T *G, const T *P, int p0,
 // per lane.  Four LDS rows of 1,366 slots (44 KB: three workgroups per CU),
 // each intermediate in the layout its accesses favour (Plain / Pad):
 //   stage     B0 = P6 (U, Plain), B1 = G3 (dY, Plain)
-//   round 1   B2 = P5 = A6 . P6 (mode 6, Pad),   B3 = G4 = A3^T . G3 (mode 3, Pad)
-//   round 2   B0 = P4 = A5 . P5 (mode 5, Pad),   B1 = G5 = A4^T . G4 (mode 4, Pad)
+//   step 1    B2 = P5 = A6 . P6 (mode 6, Pad),   B3 = G4 = A3^T . G3 (mode 3, Pad)
+//   step 2    B0 = P4 = A5 . P5 (mode 5, Pad),   B1 = G5 = A4^T . G4 (mode 4, Pad)
 //   grams     dA_4 (G4, P4), dA_5 (G5, P5)
-//   round 3   B2 = P3 = A4 . P4 (mode 4, Plain), B3 = G6 = A5^T . G5 (mode 5, Plain)
+//   step 3    B2 = P3 = A4 . P4 (mode 4, Plain), B3 = G6 = A5^T . G5 (mode 5, Plain)
 //   restage   B0 = U, B1 = dY again, from the registers they were staged from
 //   grams     dA_3 (G3, P3), dA_6 (G6, P6)
 // The next item's rows are loaded into those registers after the restage.
