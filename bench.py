@@ -714,7 +714,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py: measures one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "xbatched_d6n4":

@@ -67,7 +67,7 @@ typedef enum kron_status {
     KRON_EINVAL = 1,      /* invalid argument (d, n, sizes, null pointers) */
     KRON_EWORKSPACE = 2,  /* workspace (W / pW / Wcapacity) too small */
     KRON_EHIP = 3,        /* HIP runtime reported an error */
-    KRON_ENOTSUP = 4,     /* shape outside every kernel in this build */
+    KRON_ENOTSUP = 4,     /* shape outside every compiled kernel */
     KRON_ETOOBIG = 5      /* batchCount above KRON_NOALLOC_MAX_BATCH */
 } kron_status;
 

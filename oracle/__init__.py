@@ -9,7 +9,7 @@ ctypes front end over two CPU libraries:
   build time.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
-this package, as the checker; the product (kronmult_amd) never does.
+this package, to check results against; the product (kronmult_amd) never does.
 
 Arrays are numpy, column-major by the reference's convention: a batch of
 vectors is a C-contiguous (batch, size) array, i.e. (size, batch) in

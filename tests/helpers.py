@@ -1,7 +1,7 @@
 """Shared test helpers: seeded inputs, reference closed-form inputs, checkers.
 
 All randomness comes from splitmix64 (vectorised here, and in C in
-oracle/kron_oracle.c; the two are pinned against each other), so the GPU box
+oracle/kron_oracle.c; the two are pinned against each other), so any machine
 regenerates exactly the inputs the fixtures were made from.
 """
 import numpy as np
