@@ -112,5 +112,9 @@ def test_adjoint_workspace_query(lib):
     assert n_out.value == 2 * 6 ** 6
     assert lib.kron_batched_adjoint_workspace(2, 3, 8, ctypes.byref(n_out)) == 0
     assert n_out.value == 0
+    # d = 6, n = 4 (4,096 elements) runs on chip in both precisions
+    for elem in (8, 4):
+        assert lib.kron_batched_adjoint_workspace(6, 4, elem, ctypes.byref(n_out)) == 0
+        assert n_out.value == 0
     assert lib.kron_batched_adjoint_workspace(7, 2, 8, ctypes.byref(n_out)) == 1
     assert lib.kron_batched_adjoint_workspace(3, 2, 2, ctypes.byref(n_out)) == 1
